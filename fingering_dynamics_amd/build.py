@@ -1,0 +1,65 @@
+"""Build the native libraries in-tree.
+
+libfdlbm.so     HIP kernels + C ABI for gfx950 (hipcc)
+libfdnode.so    host FP64 build of the per-node arithmetic (g++), loaded only
+                by the CPU tests
+
+Run ``python -m fingering_dynamics_amd.build`` (or ``__graft_entry__.build()``).
+Every call compiles: a stale library is never reused.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "_lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 VALU ops are disabled for every HIP translation unit.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + INCLUDE, "-I" + CSRC]
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if proc.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build_hip():
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libfdlbm.so")
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "fdlbm.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_node_host():
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libfdnode.so")
+    tmp = out + ".tmp"
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + CSRC,
+          os.path.join(CSRC, "node_host.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    return [build_hip(), build_node_host()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)
+    sys.exit(0)

@@ -1,0 +1,167 @@
+"""Drop-in for ``Compute`` of lattice_boltzmann/fingering_periodic.py.
+
+The reference driver does
+
+    cm = Compute(mask)                                   # :452
+    for i in range(MAX_T):                               # :454-479
+        <collide>; stream(cm.f, cm.g); bb.halfway_bounceback_...(...)
+        cm.zou_he_boundary_inlet(); cm.zou_he_boundary_outlet(); <moments>
+    plt.pcolor(x, y, cm.psi)                             # :486-487
+
+With this class the loop body is one call, ``cm.step(MAX_T)``, that runs
+the fused gfx950 kernel through libfdlbm.so; ``cm.psi``, ``cm.rho``,
+``cm.ux``, ``cm.uy``, ``cm.f`` and ``cm.g`` read the state back with the
+reference's shapes (psi, f, g full; rho, ux, uy compressed over the fluid
+mask in C order).
+
+There is no CPU fallback: constructing a Compute without the built library
+or without a gfx950 device raises.
+"""
+import numpy as np
+
+from . import _native
+from .params import LbmParams
+
+
+def _to_fd_params(p: LbmParams) -> _native.FdParams:
+    return _native.FdParams(
+        H=p.H, W=p.W, tau=p.tau, eta_n=p.eta_n, M=p.M, a=p.a, kappa=p.kappa, gamma=p.gamma,
+        psi_wall=p.psi_wall, u0=p.u0, rho_out=p.rho_out, outlet_f3_coef=p.outlet_f3_coef,
+        gx=p.gx, gy=p.gy, y_bc=p.y_bc, x_bc=p.x_bc, profile=p.profile, outlet=p.outlet,
+        init_mode=p.init_mode, wall_lo=p.wall_lo, wall_hi=p.wall_hi_row)
+
+
+def initial_psi(params: LbmParams, mask):
+    """psi = -1, +1 in the first n_inv_cols columns, psi_wall on solids
+    (fingering_periodic.py:90-93)."""
+    psi = np.full((params.H, params.W), -1.0)
+    psi[:, :params.n_inv_cols] = 1.0
+    psi[~np.asarray(mask, dtype=bool)] = params.psi_wall
+    return psi
+
+
+class Compute:
+    """GPU solver state for one H x W domain.
+
+    mask    bool[H][W], True on fluid nodes (fingering_periodic.py:450-451)
+    links   uint8[H][W] bounce-back bits (geometry.Bounce_back.link_bits_*)
+    params  LbmParams
+    psi0    optional initial psi [H][W] (default: initial_psi)
+    rho0    optional initial rho [H][W] (default: params.rho0 on fluid)
+    """
+
+    def __init__(self, mask, links, params: LbmParams, psi0=None, rho0=None):
+        self._lib = _native.load()
+        self.params = params
+        self.mask = np.ascontiguousarray(mask, dtype=bool)
+        H, W = self.mask.shape
+        if (H, W) != (params.H, params.W):
+            raise ValueError("mask shape %s != params (%d, %d)" % (self.mask.shape, params.H, params.W))
+        self.H, self.W = H, W
+        solid = np.ascontiguousarray(~self.mask, dtype=np.uint8)
+        bits = np.ascontiguousarray(links, dtype=np.uint8)
+        if bits.shape != (H, W):
+            raise ValueError("links must be uint8[H][W]")
+        self._fdp = _to_fd_params(params)
+        h = _native.ctypes.c_void_p()
+        _native.check(self._lib.fd_create(_native.ctypes.byref(self._fdp), _native.ptr(solid),
+                                          _native.ptr(bits), _native.ctypes.byref(h)))
+        self._h = h
+        if psi0 is None:
+            psi0 = initial_psi(params, self.mask)
+        if rho0 is None:
+            rho0 = np.full((H, W), float(params.rho0))
+        self.init(psi0, rho0)
+
+    # ------------------------------------------------------------ control
+    def init(self, psi0, rho0=None):
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+        rho0 = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
+        _native.check(self._lib.fd_init(self._h, _native.ptr(psi0), _native.ptr(rho0)))
+        self._cache = None
+
+    def step(self, n=1):
+        """Run n iterations of fingering_periodic.py:455-479 (asynchronous)."""
+        _native.check(self._lib.fd_step(self._h, int(n)))
+        self._cache = None
+
+    @property
+    def t(self):
+        return int(self._lib.fd_steps_done(self._h))
+
+    def synchronize(self):
+        _native.check(self._lib.fd_synchronize(self._h))
+
+    def last_step_ms(self):
+        return float(self._lib.fd_last_step_ms(self._h))
+
+    def set_rows_per_wave(self, rows):
+        _native.check(self._lib.fd_set_rows_per_wave(self._h, int(rows)))
+
+    # ------------------------------------------------------------ outputs
+    def fields(self):
+        """(psi, rho, ux, uy) as float32 [H][W] (rho/u are 0 on solid nodes)."""
+        if self._cache is None:
+            out = [np.empty((self.H, self.W), dtype=np.float32) for _ in range(4)]
+            _native.check(self._lib.fd_get_fields(self._h, *[_native.ptr(a) for a in out]))
+            self._cache = tuple(out)
+        return self._cache
+
+    @property
+    def psi(self):
+        return self.fields()[0].astype(np.float64)
+
+    @property
+    def rho(self):
+        return self.fields()[1][self.mask].astype(np.float64)
+
+    @property
+    def ux(self):
+        return self.fields()[2][self.mask].astype(np.float64)
+
+    @property
+    def uy(self):
+        return self.fields()[3][self.mask].astype(np.float64)
+
+    def populations(self):
+        f = np.empty((9, self.H, self.W), dtype=np.float32)
+        g = np.empty((9, self.H, self.W), dtype=np.float32)
+        _native.check(self._lib.fd_get_populations(self._h, _native.ptr(f), _native.ptr(g)))
+        return f, g
+
+    @property
+    def f(self):
+        return self.populations()[0].astype(np.float64)
+
+    @property
+    def g(self):
+        return self.populations()[1].astype(np.float64)
+
+    def mass(self):
+        """FP64 sums over fluid nodes: (sum rho, sum psi, non-finite count)."""
+        out = np.zeros(3, dtype=np.float64)
+        _native.check(self._lib.fd_phase_mass(self._h, _native.ptr(out)))
+        return float(out[0]), float(out[1]), int(out[2])
+
+    # --------------------------------------------------------- checkpoint
+    def save_state(self):
+        n = int(self._lib.fd_state_bytes(self._h))
+        buf = np.empty(n, dtype=np.uint8)
+        _native.check(self._lib.fd_save_state(self._h, _native.ptr(buf), n))
+        return buf
+
+    def load_state(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        _native.check(self._lib.fd_load_state(self._h, _native.ptr(buf), buf.size))
+        self._cache = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,156 @@
+/*
+ * fdlbm.h - C ABI of libfdlbm.so, the MI355X (gfx950) two-phase D2Q9
+ * lattice-Boltzmann step of ebinan92/Fingering_dynamics.
+ *
+ * The reference has no FFI: its boundary is the Python object surface that
+ * main() drives (lattice_boltzmann/fingering_periodic.py:369-479).  Each
+ * entry point below replaces one piece of that surface; the reference
+ * file:line it stands in for is given beside it.  Plain pointers and sizes
+ * only; every host buffer is caller-owned, C-contiguous and copied.
+ *
+ * Errors: every int-returning call returns FD_OK (0) on success and a
+ * negative FD_E* code otherwise; fd_last_error() then holds a message for
+ * the calling thread.  No C++ exception crosses this boundary.
+ * Threading: one handle per host thread; calls on a handle are serialised
+ * on the handle's HIP stream.
+ *
+ * Array conventions (the reference's): row-major [y][x], y in [0, H) with
+ * "north" = y+1, x in [0, W); populations are [9][H][W] in the D2Q9 order
+ * e0=(0,0) e1=(1,0) e2=(0,1) e3=(-1,0) e4=(0,-1) e5=(1,1) e6=(-1,1)
+ * e7=(-1,-1) e8=(1,-1) (fingering_periodic.py:50-89).
+ */
+#ifndef FDLBM_H
+#define FDLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FD_OK 0
+#define FD_E_ARG -1      /* invalid argument (shape, null pointer, range)      */
+#define FD_E_HIP -2      /* a HIP runtime call failed                           */
+#define FD_E_STATE -3    /* call not valid in the handle's current state        */
+#define FD_E_NODEV -4    /* no usable gfx950 device                             */
+
+/* y boundary */
+#define FD_Y_PERIODIC 0  /* np.roll wrap, fingering_periodic.py:219, 327-343   */
+#define FD_Y_WALLS 1     /* half-way walls + psi_wall ghost rows, fingering.py  */
+/* x boundary */
+#define FD_X_ZOUHE 0     /* Zou-He inlet x=0 / outlet x=W-1, ghost columns +-1  */
+#define FD_X_PERIODIC 1  /* build-original: periodic x                          */
+/* inlet/outlet velocity profile */
+#define FD_PROFILE_GAUSS 0 /* u0*exp(-t^2/2), fingering_periodic.py:270-271     */
+#define FD_PROFILE_CONST 1 /* u0, fingering.py:299                              */
+/* outlet kind */
+#define FD_OUTLET_VELOCITY 0 /* fingering_periodic.py:305-324                   */
+#define FD_OUTLET_PRESSURE 1 /* build-original: prescribed rho_out              */
+/* initial-state quirk */
+#define FD_INIT_MU0 0    /* mu = 0 at init, fingering_periodic.py:111-121       */
+#define FD_INIT_WALLED 1 /* mu and uy computed at init, fingering.py:121-127    */
+
+/* Node flag bits (one uint16 per node, fd_create's `links` argument uses the
+ * low 8): bit (i-1) set => direction i (1..8) is overwritten at this node by
+ * the same node's post-collision opposite population (half-way bounce-back,
+ * bounce_back.py:25-167). */
+#define FD_LINK_BITS 0x00FF
+
+/* Lattice constants and boundary selection.  Module globals of
+ * fingering_periodic.py:15-44 that Compute reads at call time. */
+typedef struct fd_params {
+    int32_t H, W;          /* rows, columns                                     */
+    double tau;            /* g relaxation time (fingering_periodic.py:26)      */
+    double eta_n;          /* lattice viscosity of the psi=+1 fluid (:34)       */
+    double M;              /* viscosity ratio eta(psi=-1)/eta(psi=+1) (:22)     */
+    double a, kappa;       /* free-energy constants (:38-39)                    */
+    double gamma;          /* mobility (:40)                                    */
+    double psi_wall;       /* wall order parameter (:18)                        */
+    double u0;             /* inlet velocity amplitude (:36)                    */
+    double rho_out;        /* pressure-outlet density (FD_OUTLET_PRESSURE)      */
+    double outlet_f3_coef; /* 2/3 (fingering_periodic.py:317) or 1.5 (fingering.py:378) */
+    double gx, gy;         /* build-original body force added to mu*grad psi    */
+    int32_t y_bc;          /* FD_Y_*                                            */
+    int32_t x_bc;          /* FD_X_*                                            */
+    int32_t profile;       /* FD_PROFILE_*                                      */
+    int32_t outlet;        /* FD_OUTLET_*                                       */
+    int32_t init_mode;     /* FD_INIT_*                                         */
+    int32_t wall_lo;       /* FD_Y_WALLS: row of the bottom wall (0, or 1 as fingering.py:573 ships) */
+    int32_t wall_hi;       /* FD_Y_WALLS: row of the top wall (H-1, or H-2)     */
+} fd_params;
+
+typedef struct fd_handle fd_handle;
+
+/* Create a solver for one H x W domain on the current HIP device.
+ * Replaces Compute(mask) construction, fingering_periodic.py:48-121, together
+ * with the mask set-up of :449-451 and the bounce-back mask lists of
+ * bounce_back.py:89-167 / :25-86 (packed per node).
+ *   solid  uint8[H*W], non-zero on solid nodes (block_psi_all == 1)
+ *   links  uint8[H*W], bit (i-1) = direction i bounced at this node
+ * The state is not initialised until fd_init. */
+int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd_handle** out);
+
+/* Initial state, fingering_periodic.py:90-121 (or fingering.py:95-127 with
+ * FD_INIT_WALLED).  psi0 double[H*W] (values on solid nodes are replaced by
+ * psi_wall); rho0 double[H*W] or NULL for rho = 1 on fluid nodes.  Resets
+ * the step counter to 0. */
+int fd_init(fd_handle* h, const double* psi0, const double* rho0);
+
+/* Advance nsteps iterations of the main loop body
+ * (fingering_periodic.py:454-479: collide, stream, bounce-back, walls,
+ * Zou-He, moments).  Enqueued on the handle's stream; returns without
+ * waiting. */
+int fd_step(fd_handle* h, int32_t nsteps);
+
+/* Number of iterations done since fd_init. */
+int64_t fd_steps_done(const fd_handle* h);
+
+/* Macroscopic fields after the last completed iteration, float[H*W] each;
+ * any pointer may be NULL.  Replaces the attributes cm.psi, cm.rho, cm.ux,
+ * cm.uy read by fingering_periodic.py:486-487 (the reference keeps rho/ux/uy
+ * compressed over fluid nodes; here solid nodes hold 0 and psi holds
+ * psi_wall there).  Synchronises the handle's stream. */
+int fd_get_fields(fd_handle* h, float* psi, float* rho, float* ux, float* uy);
+
+/* Post-stream populations f, g float[9*H*W] after the last completed
+ * iteration (cm.f, cm.g).  Either pointer may be NULL.  Synchronises. */
+int fd_get_populations(fd_handle* h, float* f, float* g);
+
+/* FP64 sums over fluid nodes of the last completed iteration:
+ * out[0] = sum rho, out[1] = sum psi, out[2] = number of fluid nodes whose
+ * rho or psi is not finite.  No reference counterpart: the reference's only
+ * health check is np.seterr(all='raise') (fingering_periodic.py:497).
+ * Synchronises. */
+int fd_phase_mass(fd_handle* h, double out[3]);
+
+/* Opaque resumable state (post-collision populations, Zou-He side buffer,
+ * step counter).  fd_state_bytes gives the size; fd_save_state copies it
+ * out, fd_load_state copies it back into a handle created with the same
+ * fd_params and masks.  No reference counterpart (the reference keeps no
+ * checkpoints, fingering_periodic.py:461-462). */
+int64_t fd_state_bytes(const fd_handle* h);
+int fd_save_state(fd_handle* h, void* buf, int64_t nbytes);
+int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes);
+
+/* Wait for the handle's queued work. */
+int fd_synchronize(fd_handle* h);
+
+/* Average device time per iteration (ms) of the last fd_step call, measured
+ * with HIP events on the handle's stream; < 0 if unavailable. */
+double fd_last_step_ms(const fd_handle* h);
+
+/* Kernel launch shape: rows per wave-strip (>= 1).  Used by the tile
+ * autotuner; the default suits large domains. */
+int fd_set_rows_per_wave(fd_handle* h, int32_t rows);
+
+/* Release every device buffer of the handle.  NULL is ignored. */
+void fd_destroy(fd_handle* h);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDLBM_H */

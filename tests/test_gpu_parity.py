@@ -1,0 +1,211 @@
+"""GPU parity: the fused gfx950 kernel (through the C ABI) against the FP64 oracle.
+
+Tolerances (FP32 kernel vs FP64 oracle, absolute, on every node):
+  populations f, g and psi, rho      2e-5   for runs up to 50 iterations
+  ux, uy                             2e-6
+These are ~100x the FP32 rounding of O(1) populations (6e-8 per operation,
+a few hundred operations per node per iteration, errors growing with the
+iteration count); an indexing, sign or boundary mistake shows up as 1e-3 or
+more within one or two iterations.
+"""
+import numpy as np
+import pytest
+
+from fingering_dynamics_amd.params import (LbmParams, X_PERIODIC, PROFILE_CONST, OUTLET_PRESSURE)
+from tests.helpers import rect_domain, circle_domain, oracle_for, oracle_fields, max_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_POP = 2e-5
+TOL_U = 2e-6
+
+
+def _compute(*a, **k):
+    from fingering_dynamics_amd.lbm import Compute
+    return Compute(*a, **k)
+
+
+def _compare(cm, o, mask, tag, pops=True):
+    psi, rho, ux, uy = cm.fields()
+    opsi, orho, oux, ouy = oracle_fields(o)
+    e = dict(psi=max_err(psi, opsi), rho=max_err(rho, orho, mask),
+             ux=max_err(ux, oux, mask), uy=max_err(uy, ouy, mask))
+    if pops:
+        f, g = cm.populations()
+        e["f"] = max_err(f, o.f)
+        e["g"] = max_err(g, o.g)
+    print(tag, {k: "%.2e" % v for k, v in e.items()}, flush=True)
+    assert e["psi"] < TOL_POP and e["rho"] < TOL_POP, (tag, e)
+    assert e["ux"] < TOL_U and e["uy"] < TOL_U, (tag, e)
+    if pops:
+        assert e["f"] < TOL_POP and e["g"] < TOL_POP, (tag, e)
+    return e
+
+
+def _run_pair(params, mask, bits, links, checkpoints, psi0=None, rho0=None):
+    cm = _compute(mask, bits, params, psi0=psi0, rho0=rho0)
+    o = oracle_for(params, mask, links, psi0=psi0, rho0=rho0)
+    done = 0
+    _compare(cm, o, mask, "t=0")
+    for t in checkpoints:
+        cm.step(t - done)
+        o.step(t - done)
+        done = t
+        _compare(cm, o, mask, "t=%d" % t)
+    cm.close()
+
+
+CASES = {
+    # fingering_periodic.py defaults (M=20, psi_wall=-0.5, Gaussian inlet)
+    "periodic_ref": dict(kind="rect", params=dict(M=20.0, psi_wall=-0.5)),
+    # cfg1-like: M=5, psi_wall=0, psi=+1 in the first columns
+    "periodic_cfg1": dict(kind="rect", params=dict(M=5.0, psi_wall=0.0, n_inv_cols=12)),
+    "periodic_const": dict(kind="rect", params=dict(M=10.0, psi_wall=0.3, profile=PROFILE_CONST)),
+    "periodic_circles": dict(kind="circle", params=dict(M=10.0, psi_wall=-0.3)),
+    "pressure_outlet": dict(kind="rect", params=dict(M=5.0, psi_wall=0.0, outlet=OUTLET_PRESSURE,
+                                                      rho_out=1.0, profile=PROFILE_CONST)),
+    # fingering.py's mobility scales with H (fingering.py:43); on these
+    # 34-row domains it is 10x smaller than on the reference's 380 rows and
+    # the walled channel goes unstable, so the walled cases use the value
+    # gamma takes at H = 380.
+    "walled_shipped": dict(kind="rect", walled=True, params=dict(M=20.0, psi_wall=-1.0, gamma=1.0027)),
+    "walled_edge": dict(kind="rect", walled=True, edge=True, params=dict(M=10.0, psi_wall=-0.5, gamma=1.0027)),
+    "walled_pressure": dict(kind="circle", walled=True, edge=True,
+                            params=dict(M=10.0, psi_wall=0.0, outlet=OUTLET_PRESSURE,
+                                        outlet_f3_coef=2.0 / 3.0, gamma=1.0027)),
+    "periodic_x_force": dict(kind="rect", walled=True, edge=True,
+                             params=dict(M=5.0, psi_wall=0.0, x_bc=X_PERIODIC, gx=1e-5, n_inv_cols=12,
+                                         gamma=1.0027)),
+}
+
+
+def _setup(case, H=34, W=70):
+    spec = CASES[case]
+    if spec["kind"] == "rect":
+        mask, bits, links = rect_domain(H, W, 6, size=4, seed=7, x_min=14)
+    else:
+        mask, bits, links = circle_domain(H, W, 3, 5, 0.88, seed=5, x_lo=12)
+    if spec.get("walled"):
+        params = LbmParams.walled(H, W, shipped_walls=not spec.get("edge", False), **spec["params"])
+    else:
+        params = LbmParams.periodic(H, W, **spec["params"])
+    return params, mask, bits, links
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_parity_small(case):
+    params, mask, bits, links = _setup(case)
+    _run_pair(params, mask, bits, links, [1, 2, 5, 20, 50])
+
+
+def test_parity_non_multiple_shapes():
+    # W not a multiple of the strip width, odd row chunks, several strips
+    for H, W in ((38, 131), (66, 300)):
+        mask, bits, links = rect_domain(H, W, 10, size=5, seed=3, x_min=20)
+        params = LbmParams.periodic(H, W, M=5.0, psi_wall=0.0, n_inv_cols=15)
+        _run_pair(params, mask, bits, links, [3, 17])
+
+
+def test_parity_rows_per_wave_sweep():
+    H, W = 40, 90
+    mask, bits, links = rect_domain(H, W, 8, size=4, seed=11, x_min=15)
+    params = LbmParams.periodic(H, W, M=10.0, psi_wall=-0.2, n_inv_cols=10)
+    o = oracle_for(params, mask, links)
+    o.step(9)
+    ref = oracle_fields(o)
+    for rows in (1, 3, 7, 16, 40, 64):
+        cm = _compute(mask, bits, params)
+        cm.set_rows_per_wave(rows)
+        cm.step(9)
+        psi, rho, ux, uy = cm.fields()
+        assert max_err(psi, ref[0]) < TOL_POP, rows
+        assert max_err(ux, ref[2], mask) < TOL_U, rows
+        cm.close()
+
+
+def test_mass_matches_oracle():
+    params, mask, bits, links = _setup("periodic_cfg1")
+    cm = _compute(mask, bits, params)
+    o = oracle_for(params, mask, links)
+    for t in (0, 10, 40):
+        if t:
+            cm.step(t - o.t)
+            o.step(t - o.t)
+        s_rho, s_psi, bad = cm.mass()
+        r_rho, r_psi = o.mass()
+        assert bad == 0
+        assert abs(s_rho - r_rho) < 1e-6 * abs(r_rho) + 1e-4, (t, s_rho, r_rho)
+        assert abs(s_psi - r_psi) < 1e-4 * mask.sum() * 1e-2 + 1e-4, (t, s_psi, r_psi)
+    cm.close()
+
+
+def test_deterministic_and_checkpoint_resume():
+    params, mask, bits, links = _setup("periodic_circles")
+    a = _compute(mask, bits, params)
+    a.step(30)
+    fa = a.fields()
+    b = _compute(mask, bits, params)
+    b.step(13)
+    blob = b.save_state()
+    b.close()
+    c = _compute(mask, bits, params)
+    c.load_state(blob)
+    assert c.t == 13
+    c.step(17)
+    fc = c.fields()
+    for x, y in zip(fa, fc):
+        assert np.array_equal(x, y)
+    a2 = _compute(mask, bits, params)
+    a2.step(30)
+    for x, y in zip(fa, a2.fields()):
+        assert np.array_equal(x, y)
+    for h in (a, c, a2):
+        h.close()
+
+
+def test_translation_equivariance_full_size():
+    """cfg2 size: rolling the geometry and the initial field along the
+    periodic y axis must roll the result bit-for-bit."""
+    H, W = 2048, 4096
+    rng = np.random.default_rng(0)
+    mask = np.ones((H, W), dtype=bool)
+    bits = np.zeros((H, W), dtype=np.uint8)
+    from fingering_dynamics_amd import geometry
+    circles = geometry.random_circles(H, W, 8, 24, 0.9, seed=2, x_lo=60)
+    bpa, side, cave, vex = geometry.Createblock(H, W).setCirleblock(circles)
+    mask = geometry.fluid_mask(bpa)
+    bits = geometry.Bounce_back(H, W).link_bits_circle(side, cave, vex)
+    params = LbmParams.periodic(H, W, M=10.0, psi_wall=0.0, profile=PROFILE_CONST, n_inv_cols=40)
+    psi0 = np.where(mask, np.where(np.arange(W)[None, :] < 40, 1.0, -1.0), 0.0)
+    psi0 = psi0 + 0.01 * rng.standard_normal((H, W)) * mask
+    shift = 777
+    a = _compute(mask, bits, params, psi0=psi0)
+    b = _compute(np.roll(mask, shift, 0), np.roll(bits, shift, 0), params, psi0=np.roll(psi0, shift, 0))
+    a.step(12)
+    b.step(12)
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(np.roll(x, shift, 0), y)
+    ma, mb = a.mass(), b.mass()
+    assert abs(ma[0] - mb[0]) < 1e-9 * abs(ma[0])
+    a.close()
+    b.close()
+
+
+def test_closed_domain_conservation_full_size():
+    """Periodic in x and y, no obstacles, no force: sum rho and sum psi are
+    invariants of the exact update; FP32 drift must stay tiny."""
+    H, W = 2048, 4096
+    params = LbmParams.periodic(H, W, M=10.0, psi_wall=0.0, x_bc=X_PERIODIC)
+    rng = np.random.default_rng(1)
+    yy, xx = np.mgrid[0:H, 0:W]
+    psi0 = np.tanh((np.hypot(yy - H / 2, xx - W / 2) - 300.0) / 2.0) + 0.01 * rng.standard_normal((H, W))
+    mask = np.ones((H, W), dtype=bool)
+    cm = _compute(mask, np.zeros((H, W), np.uint8), params, psi0=psi0)
+    r0, p0, bad0 = cm.mass()
+    cm.step(200)
+    r1, p1, bad1 = cm.mass()
+    print("closed domain drift", (r1 - r0) / r0, (p1 - p0) / (H * W))
+    assert bad0 == 0 and bad1 == 0
+    assert abs(r1 - r0) < 1e-5 * abs(r0)
+    assert abs(p1 - p0) < 1e-5 * H * W
+    cm.close()
