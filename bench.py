@@ -1,0 +1,190 @@
+"""Throughput of the fused two-phase D2Q9 step on one MI355X.
+
+Workload (the single-domain configuration): 4096 x 2048 nodes (W x H),
+Zou-He velocity inlet u = 0.001, pressure outlet rho = 1, half-way walls on
+the bottom and top rows, seeded circular obstacles of radius 8..24 to
+porosity 0.75 (seed 2), viscosity ratio M = 10, FP32 populations.  A step is
+one iteration of the hot loop of lattice_boltzmann/fingering_periodic.py
+(collide, stream, bounce-back, walls, Zou-He, moments) over the whole
+domain; the populations stay resident in HBM.
+
+Prints ONE JSON line.  ``value`` is MLUPS over all H*W nodes.  ``roofline``
+prices the step kernel at 144 algorithmic bytes per fluid-node update
+(9 f + 9 g FP32 read and written) against 8 TB/s, using the kernel's average
+launch time from HIP events on the solver's stream.  ``cpu_baseline`` times
+the FP64 numpy restatement (the oracle, one core) on a few iterations of the
+same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+"""
+import argparse
+import json
+import os
+import platform
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+H, W = 2048, 4096
+BYTES_PER_FLUID_UPDATE = 144          # 9 f + 9 g, FP32, read + write
+HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E
+PSI_WALL = -0.5                       # the reference's wall value (fingering_periodic.py:18)
+U_IN = 0.001                          # inlet velocity (see workload())
+
+
+def workload():
+    """Masks, link bits and parameters of the 4096 x 2048 channel.
+
+    Zou-He inlet with a constant velocity, pressure outlet rho = 1, half-way
+    walls on rows 0 and H-1, circles of radius 8..24 to porosity 0.75 kept
+    64 columns clear of the inlet and outlet, M = 10, psi = +1 in the first
+    40 columns.  Bounce-back uses the complete half-way link set (every solid
+    -> fluid link), so solid nodes are inert.  The inlet velocity is 0.001:
+    at 0.01 the Darcy pressure drop across 4096 columns of this bed
+    (about 3 nu L u / k in density units, ~3) is far beyond what the weakly
+    compressible scheme carries, and every variant tried diverged within
+    2000-7000 iterations; the step's arithmetic and memory traffic do not
+    depend on these values."""
+    from fingering_dynamics_amd import geometry
+    from fingering_dynamics_amd.params import LbmParams, Y_WALLS, PROFILE_CONST, OUTLET_PRESSURE
+
+    circles = geometry.random_circles(H, W, 8, 24, 0.75, seed=2, margin=2, x_lo=64, x_hi=W - 65)
+    bpa, _, _, _ = geometry.Createblock(H, W).setCirleblock(circles)
+    mask = geometry.fluid_mask(bpa)
+    bits = geometry.link_bits_complete(mask)
+    params = LbmParams.periodic(
+        H, W, M=10.0, psi_wall=PSI_WALL, u0=U_IN, profile=PROFILE_CONST,
+        outlet=OUTLET_PRESSURE, rho_out=1.0, y_bc=Y_WALLS, wall_lo=0, wall_hi=H - 1,
+        n_inv_cols=40)
+    return mask, bits, params, circles
+
+
+def traffic_from_profiles():
+    """HBM bytes per step launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE), as recorded in profiles/; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "step_kernel_4096x2048_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+CPU_WINDOW = (512, 1024)              # rows, columns of the CPU-baseline sample
+
+
+def cpu_baseline(mask, bits, params, steps):
+    """The FP64 oracle (numpy, one core) on `steps` iterations of the
+    bottom-left 1024 x 512 window of the same workload (same obstacles,
+    inlet, walls and parameters; the full domain costs about a minute per
+    iteration in numpy).  Its bounce-back masks are the unpacked link bits
+    (the geometry itself is not what is timed)."""
+    from oracle import lbm64
+    from fingering_dynamics_amd.geometry import unpack_links
+    h, w = CPU_WINDOW
+    m = np.ascontiguousarray(mask[:h, :w])
+    b = np.ascontiguousarray(bits[:h, :w])
+    p = params.with_(H=h, W=w, wall_lo=0, wall_hi=h - 1)
+    o = lbm64.Lbm64.from_params(p, m, unpack_links(b))
+    o.step(1)                                    # first iteration pays page faults
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(h * w * steps / dt / 1e6, 4),
+        "unit": "MLUPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": "%d iterations of the %dx%d bottom-left window of the workload, FP64 numpy "
+                  "restatement of fingering_periodic.py:454-479, %.1f s (host %s, %d cpus visible, "
+                  "numpy single-threaded)" % (steps, w, h, dt, platform.processor() or platform.machine(),
+                                              os.cpu_count() or 0),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=15,
+                    help="iterations of the CPU baseline sample (0 skips it)")
+    ap.add_argument("--tile", default="", help="launch shape WX,ROWS (default: the library's choice)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    # single process: any launcher-provided rank other than 0 has nothing to do
+    if int(os.environ.get("RANK", "0")) != 0:
+        sys.exit(0)
+
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ.setdefault(k, "1")
+
+    from fingering_dynamics_amd.lbm import Compute
+    mask, bits, params, _ = workload()
+    n_fluid = int(mask.sum())
+    cm = Compute(mask, bits, params)
+    if args.tile:
+        wx, rows = (int(v) for v in args.tile.split(","))
+        cm.set_tile(wx, rows)
+    cm.step(args.warmup)
+    cm.synchronize()
+    t0 = time.perf_counter()
+    cm.step(args.steps)
+    cm.synchronize()
+    dt = time.perf_counter() - t0
+    kernel_ms = cm.last_step_ms()
+    s_rho, s_psi, bad = cm.mass()
+    cm.close()
+
+    ms_per_step = dt * 1e3 / args.steps
+    mlups = H * W * args.steps / dt / 1e6
+    algo_bytes = BYTES_PER_FLUID_UPDATE * n_fluid
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+        "traffic": traffic_from_profiles(),
+        "kernel_ms": round(kernel_ms, 5),
+        "algorithmic_bytes_per_launch": algo_bytes,
+        "fluid_nodes": n_fluid,
+    }
+    cpu = cpu_baseline(mask, bits, params, args.cpu_steps) if args.cpu_steps > 0 else None
+    out = {
+        "metric": "MLUPS (all H*W nodes) of the fused two-phase D2Q9 step; % of HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded circular obstacles (seed 2), initial psi=+1 in the first 40 columns",
+        "config": {
+            "workload": "single domain 4096x2048 (W x H), Zou-He velocity inlet (u=%g), pressure outlet, "
+                        "half-way walls, seeded circles r 8-24 to porosity 0.75, M=10, psi_wall=%g" % (U_IN, PSI_WALL),
+            "H": H, "W": W, "porosity": round(n_fluid / (H * W), 4), "M": params.M,
+            "psi_wall": params.psi_wall,
+        },
+        "mflups": round(n_fluid * args.steps / dt / 1e6, 1),
+        "mass_check": {"sum_rho": s_rho, "sum_psi": s_psi, "non_finite": bad},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -34,6 +34,7 @@ EXPORTS = {
     "fd_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_steps_done": (ctypes.c_int64, [ctypes.c_void_p]),
+    "fd_solid_inert": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_get_fields": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "fd_get_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_phase_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -42,7 +43,7 @@ EXPORTS = {
     "fd_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "fd_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_last_step_ms": (ctypes.c_double, [ctypes.c_void_p]),
-    "fd_set_rows_per_wave": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fd_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fd_destroy": (None, [ctypes.c_void_p]),
     "fd_last_error": (ctypes.c_char_p, []),
 }

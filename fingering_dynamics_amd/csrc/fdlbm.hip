@@ -15,8 +15,8 @@
 //
 // HBM layout: one allocation per ping-pong set holding 18 planes
 // (f0..f8, g0..g8), each [H][Wp] float with the row pitch Wp a multiple of
-// 64 floats (256 B); one uint16 flag per node (bits 0-7 bounce-back links of
-// directions 1..8, bit 8 solid); a 4 x H side buffer with P_x = d_x psi * mu
+// 64 floats (256 B); one uint32 flag word per node (slot / explicit
+// bounce-back links, solid bit); a 4 x H side buffer with P_x = d_x psi * mu
 // and P_y = d_y psi * mu of columns 0 and W-1 taken from the previous
 // iteration's psi, which the Zou-He update reads (fingering_periodic.py:273-275).
 #include <hip/hip_runtime.h>
@@ -34,7 +34,21 @@
 
 namespace {
 
-constexpr unsigned kSolid = 0x100u;
+// Per-node flags (uint32):
+//   bits 0-7   slot links: direction i is bounced back through the solid
+//              upstream node's own slot (written by this node's collision)
+//   bits 8-15  explicit links: direction i reloaded from this node's own
+//              post-collision opposite population when pulled
+//   bit 16     solid
+// In the inert-solid mode (leak-free geometry) solid nodes are never read
+// or written; otherwise every link is explicit and solid nodes stream their
+// populations on unchanged (store-and-forward, fingering_periodic.py:455-466).
+constexpr unsigned kSlotLinks = 0xFFu;
+constexpr unsigned kExplShift = 8;
+constexpr unsigned kSolid = 1u << 16;
+// A vector offset beyond any descriptor's range: such buffer loads return 0
+// and such stores are dropped, without a memory access.
+constexpr unsigned kOOB = 0x7FFFFFFCu;
 
 struct KParams {
     int H, W, Wp;
@@ -42,6 +56,7 @@ struct KParams {
     int rows;       // rows per workgroup chunk
     int nstrips;    // workgroup strips across x
     int y_walls, x_periodic, pressure, wall_lo, wall_hi;
+    int inert;      // solid nodes are never read (leak-free geometry)
     float rho_out, c3;
     fdn::Consts<float> c;
 };
@@ -49,7 +64,7 @@ struct KParams {
 struct KBufs {
     const float* __restrict__ src;     // post-collision populations (18 planes)
     float* __restrict__ dst;           // next post-collision populations / post-stream copy
-    const uint16_t* __restrict__ flags;
+    const uint32_t* __restrict__ flags;
     const float* __restrict__ side_in; // [4][H] P of the previous psi
     float* __restrict__ side_out;
     const float* __restrict__ uprof;   // [H] inlet/outlet velocity
@@ -59,113 +74,175 @@ struct KBufs {
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// Pull the 9 populations of one distribution (plane base P0 = 0 for f, 9
-// for g) into node (y, x), then apply the node's bounce-back links and the
-// walls.  All sources are post-collision values of the previous iteration.
-__device__ __forceinline__ void pull(const float* __restrict__ src, const KParams& k, int P0,
-                                     int y, int x, unsigned fl, float v[9]) {
-    const int yn = (y + 1 == k.H) ? 0 : y + 1;
-    const int ys = (y == 0) ? k.H - 1 : y - 1;
-    const int xe = (x + 1 == k.W) ? 0 : x + 1;
-    const int xw = (x == 0) ? k.W - 1 : x - 1;
-    const float* __restrict__ b = src + P0 * k.Q;
-    const int rc = y * k.Wp, rn = yn * k.Wp, rs = ys * k.Wp;
+// Row geometry shared by every lane of a wave (y is uniform across a
+// workgroup): the three source rows of a pull, with periodic wrap.
+struct Rows {
+    int s, c, n;  // y-1, y, y+1 (wrapped), already multiplied by the pitch
+};
+
+__device__ __forceinline__ Rows rows_of(const KParams& k, int y) {
+    Rows r;
+    r.c = y * k.Wp;
+    r.s = ((y == 0) ? k.H - 1 : y - 1) * k.Wp;
+    r.n = ((y + 1 == k.H) ? 0 : y + 1) * k.Wp;
+    return r;
+}
+
+// Buffer-resource access to a population set: the descriptor covers the
+// whole 18-plane allocation; the vector offset is the lane's column in bytes
+// and the scalar offset the (plane, row, column shift) part, which is the
+// same for every lane of the workgroup.  Out-of-range offsets read 0.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const float* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kRsrcWord3);
+}
+// The b32 builtins move raw bits (unsigned); the float is bit-cast, never converted.
+__device__ __forceinline__ float bld(rsrc_t r, unsigned vbyte, int sidx) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vbyte, sidx * 4, 0));
+}
+__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vbyte, int sidx) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vbyte, sidx * 4, 0);
+}
+
+// Streaming loads of one distribution (planes P0..P0+8) into node (y, x):
+// direction i comes from (y - e_iy, x - e_ix).  x - 1 and x + 1 are not
+// wrapped: at x = 0 and x = W-1 the affected directions (1, 5, 8 and 3, 6,
+// 7) are either rewritten by Zou-He or, for periodic x, reloaded by
+// pull_fix.  The -1 shifts only occur in planes >= 1, so every scalar index
+// is non-negative.
+__device__ __forceinline__ void pull_raw(rsrc_t src, const KParams& k, int P0, const Rows& r, unsigned xb,
+                                         float v[9]) {
+    const int b = P0 * k.Q;
     const int Q = k.Q;
-    v[0] = b[rc + x];
-    v[1] = b[Q + rc + xw];
-    v[2] = b[2 * Q + rs + x];
-    v[3] = b[3 * Q + rc + xe];
-    v[4] = b[4 * Q + rn + x];
-    v[5] = b[5 * Q + rs + xw];
-    v[6] = b[6 * Q + rs + xe];
-    v[7] = b[7 * Q + rn + xe];
-    v[8] = b[8 * Q + rn + xw];
-    const unsigned links = fl & FD_LINK_BITS;
-    if (links) {
+    v[0] = bld(src, xb, b + r.c);
+    v[1] = bld(src, xb, b + Q + r.c - 1);
+    v[2] = bld(src, xb, b + 2 * Q + r.s);
+    v[3] = bld(src, xb, b + 3 * Q + r.c + 1);
+    v[4] = bld(src, xb, b + 4 * Q + r.n);
+    v[5] = bld(src, xb, b + 5 * Q + r.s - 1);
+    v[6] = bld(src, xb, b + 6 * Q + r.s + 1);
+    v[7] = bld(src, xb, b + 7 * Q + r.n + 1);
+    v[8] = bld(src, xb, b + 8 * Q + r.n - 1);
+}
+
+// Periodic-x wrap for the first and last column, then the node's
+// bounce-back links and the walls; all overwrite values with the node's own
+// post-collision populations (bounce_back.py:25-167, fingering.py:432-451).
+__device__ __forceinline__ void pull_fix(rsrc_t src, const KParams& k, int P0, const Rows& r, int y,
+                                         unsigned x, unsigned fl, float v[9]) {
+    const int b = P0 * k.Q;
+    const int Q = k.Q;
+    const unsigned xb = x * 4u;
+    if (k.x_periodic) {
+        if (x == 0) {
+            const unsigned wb = (unsigned)(k.W - 1) * 4u;
+            v[1] = bld(src, wb, b + Q + r.c);
+            v[5] = bld(src, wb, b + 5 * Q + r.s);
+            v[8] = bld(src, wb, b + 8 * Q + r.n);
+        }
+        if (x == (unsigned)k.W - 1) {
+            v[3] = bld(src, 0u, b + 3 * Q + r.c);
+            v[6] = bld(src, 0u, b + 6 * Q + r.s);
+            v[7] = bld(src, 0u, b + 7 * Q + r.n);
+        }
+    }
+    // explicit bounce-back: all eight directions are loaded into temporaries
+    // (out-of-range offset for lanes without that link, so no memory access)
+    // and selected afterwards, so no load waits on an earlier one; waves
+    // without any explicit link skip the block.
+    const unsigned links = (fl >> kExplShift) & 0xFFu;
+    if (__any(links != 0u)) {
+        float t[9];
 #pragma unroll
-        for (int i = 1; i < 9; ++i)
-            if (links & (1u << (i - 1))) v[i] = b[fdn::OPP[i] * Q + rc + x];
+        for (int i = 1; i < 9; ++i) {
+            const bool on = (links >> (i - 1)) & 1u;
+            t[i] = bld(src, on ? xb : kOOB, b + fdn::OPP[i] * Q + r.c);
+        }
+#pragma unroll
+        for (int i = 1; i < 9; ++i) v[i] = ((links >> (i - 1)) & 1u) ? t[i] : v[i];
     }
     if (k.y_walls) {
         if (y == k.wall_lo) {
-            v[2] = b[4 * Q + rc + x];
-            v[5] = b[7 * Q + rc + x];
-            v[6] = b[8 * Q + rc + x];
+            v[2] = bld(src, xb, b + 4 * Q + r.c);
+            v[5] = bld(src, xb, b + 7 * Q + r.c);
+            v[6] = bld(src, xb, b + 8 * Q + r.c);
         }
         if (y == k.wall_hi) {
-            v[4] = b[2 * Q + rc + x];
-            v[7] = b[5 * Q + rc + x];
-            v[8] = b[6 * Q + rc + x];
+            v[4] = bld(src, xb, b + 2 * Q + r.c);
+            v[7] = bld(src, xb, b + 5 * Q + r.c);
+            v[8] = bld(src, xb, b + 6 * Q + r.c);
         }
     }
 }
 
-// Zou-He (and walled corner) update of g at an inlet/outlet node.
+__device__ __forceinline__ void pull(rsrc_t src, const KParams& k, int P0, int y, unsigned x, unsigned fl,
+                                     float v[9]) {
+    const Rows r = rows_of(k, y);
+    pull_raw(src, k, P0, r, x * 4u, v);
+    pull_fix(src, k, P0, r, y, x, fl, v);
+}
+
+// Zou-He (and walled corner) update of g at an inlet/outlet node.  The
+// candidate updates are computed on full copies and merged with per-element
+// selects: branches that store to different elements of one array make the
+// optimiser merge the stores through a pointer phi, which forces the array
+// out of registers.
 __device__ __forceinline__ void bc_g(const KParams& k, int y, int x, float g[9]) {
     if (k.x_periodic) return;
     if (x == 0) {
-        if (k.y_walls && y == 0)
-            fdn::inlet_corner_bottom_g(g);
-        else if (k.y_walls && y == k.H - 1)
-            fdn::inlet_corner_top_g(g);
-        else
-            fdn::inlet_g(g);
+        const bool bot = k.y_walls && y == 0;
+        const bool top = k.y_walls && y == k.H - 1;
+        float n[9], cb[9], ct[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) n[i] = cb[i] = ct[i] = g[i];
+        fdn::inlet_g(n);
+        fdn::inlet_corner_bottom_g(cb);
+        fdn::inlet_corner_top_g(ct);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = bot ? cb[i] : (top ? ct[i] : n[i]);
     }
     if (x == k.W - 1) {
         fdn::outlet_g(g);
-        if (k.y_walls) {
-            if (y == 0) g[2] = g[4];
-            if (y == k.H - 1) g[4] = g[2];
-        }
+        const float g2 = (k.y_walls && y == 0) ? g[4] : g[2];
+        const float g4 = (k.y_walls && y == k.H - 1) ? g[2] : g[4];
+        g[2] = g2;
+        g[4] = g4;
     }
 }
 
 // Inlet density of row r (walled corner nodes use the adjacent row's).
-__device__ __noinline__ float inlet_rho_row(const KParams& k, const KBufs& b, int r) {
+__device__ __forceinline__ float inlet_rho_row(const KParams& k, const KBufs& b, rsrc_t src, int r) {
     float f[9];
-    pull(b.src, k, 0, r, 0, b.flags[r * k.Wp], f);
+    pull(src, k, 0, r, 0u, b.flags[r * k.Wp], f);  // column 0 is never solid in the inert mode
     return fdn::inlet_rho(f, b.uprof[r], b.side_in[r]);
 }
 
-__device__ __forceinline__ void bc_f(const KParams& k, const KBufs& b, int y, int x, float f[9]) {
+__device__ __forceinline__ void bc_f(const KParams& k, const KBufs& b, rsrc_t src, int y, int x, float f[9]) {
     if (k.x_periodic) return;
     if (x == 0) {
-        if (k.y_walls && y == 0)
-            fdn::inlet_corner_bottom_f(f, inlet_rho_row(k, b, 1));
-        else if (k.y_walls && y == k.H - 1)
-            fdn::inlet_corner_top_f(f, inlet_rho_row(k, b, k.H - 2));
-        else
-            fdn::inlet_f(f, b.uprof[y], b.side_in[y], b.side_in[k.H + y]);
+        const bool bot = k.y_walls && y == 0;
+        const bool top = k.y_walls && y == k.H - 1;
+        float rho_nb = 0.f;
+        if (bot || top) rho_nb = inlet_rho_row(k, b, src, bot ? 1 : k.H - 2);
+        float n[9], cb[9], ct[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) n[i] = cb[i] = ct[i] = f[i];
+        fdn::inlet_f(n, b.uprof[y], b.side_in[y], b.side_in[k.H + y]);
+        fdn::inlet_corner_bottom_f(cb, rho_nb);
+        fdn::inlet_corner_top_f(ct, rho_nb);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[i] = bot ? cb[i] : (top ? ct[i] : n[i]);
     }
     if (x == k.W - 1) {
         fdn::outlet_f(f, b.uprof[y], b.side_in[2 * k.H + y], b.side_in[3 * k.H + y],
                       k.pressure != 0, k.rho_out, k.c3);
-        if (k.y_walls) {
-            if (y == 0) f[2] = f[4];
-            if (y == k.H - 1) f[4] = f[2];
-        }
+        const float f2 = (k.y_walls && y == 0) ? f[4] : f[2];
+        const float f4 = (k.y_walls && y == k.H - 1) ? f[2] : f[4];
+        f[2] = f2;
+        f[4] = f4;
     }
-}
-
-// psi of node (y, x) after pull + bounce-back + walls + Zou-He, with the
-// stencil's ghost values outside the domain: psi_wall ghost rows when the
-// channel is walled (fingering.py:224), +1 / -1 ghost columns at x = -1 / W
-// unless x is periodic (fingering_periodic.py:94-95, 218).  Also returns the
-// node's g and flags for in-domain nodes.
-__device__ __forceinline__ float psi_node(const KParams& k, const KBufs& b, int y, int x,
-                                          float g[9], unsigned& fl) {
-    fl = kSolid;
-    if (k.y_walls && (y < 0 || y >= k.H)) return k.c.psi_wall;
-    y = wrap(y, k.H);
-    if (x < 0 || x >= k.W) {
-        if (!k.x_periodic || x < -1 || x > k.W) return x < 0 ? 1.0f : -1.0f;
-        x = wrap(x, k.W);
-    }
-    fl = b.flags[y * k.Wp + x];
-    pull(b.src, k, 9, y, x, fl, g);
-    bc_g(k, y, x, g);
-    if (fl & kSolid) return k.c.psi_wall;
-    return g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
 }
 
 template <int NT>
@@ -197,9 +274,181 @@ __device__ __forceinline__ void block_sum3(double s0, double s1, double s2, doub
     }
 }
 
+// Where a lane's column sits: inside the domain, on a ghost column of the
+// psi stencil, or unused (beyond x = W in the last strip).
+struct Lane {
+    unsigned x;      // column (wrapped for periodic x), valid if inside
+    bool inside;     // pulls from memory
+    float ghost;     // psi value when !inside
+};
+
+__device__ __forceinline__ Lane lane_of(const KParams& k, int x) {
+    Lane l;
+    l.inside = true;
+    l.ghost = 0.f;
+    if (x < 0 || x >= k.W) {
+        if (k.x_periodic && x >= -1 && x <= k.W) {
+            x = (x < 0) ? x + k.W : x - k.W;
+        } else {
+            // ghost columns +1 at x = -1 and -1 at x = W (fingering_periodic.py:94-95)
+            l.inside = false;
+            l.ghost = (x < 0) ? 1.0f : -1.0f;
+        }
+    }
+    l.x = (unsigned)x;
+    return l;
+}
+
+// Flags of row y (wrapped) for this lane; ghost rows and ghost columns read
+// as solid so no pull is issued for them.
+__device__ __forceinline__ unsigned flags_of(const KParams& k, const KBufs& b, const Lane& ln, int y) {
+    if (!ln.inside || (k.y_walls && (y < 0 || y >= k.H))) return kSolid;
+    y = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
+    return b.flags[y * k.Wp + ln.x];
+}
+
+// Bounce-back through the solid neighbour's slot (inert mode): direction i
+// of node x is bounced when x - e_i is solid, so the next pull of direction
+// i at x reads plane i at x - e_i, a slot nothing else reads or writes.
+// Storing the post-collision opposite population there makes that pull
+// return f*_opp(i)(x), which is what half-way bounce-back prescribes.
+__device__ __forceinline__ void slot_bounce(const KParams& k, rsrc_t dst, int y, unsigned x, unsigned fl,
+                                            const float f[9], const float g[9]) {
+    const unsigned links = fl & kSlotLinks;
+    if (!__any(links != 0u)) return;
+#pragma unroll
+    for (int i = 1; i < 9; ++i) {
+        const bool on = (links >> (i - 1)) & 1u;
+        if (!__any(on)) continue;
+        int ys = y - fdn::EY[i];
+        ys = (ys < 0) ? ys + k.H : (ys >= k.H ? ys - k.H : ys);
+        int xs = (int)x - fdn::EX[i];
+        xs = (xs < 0) ? xs + k.W : (xs >= k.W ? xs - k.W : xs);
+        const unsigned vb = on ? (unsigned)xs * 4u : kOOB;
+        bst(f[fdn::OPP[i]], dst, vb, i * k.Q + ys * k.Wp);
+        bst(g[fdn::OPP[i]], dst, vb, (9 + i) * k.Q + ys * k.Wp);
+    }
+}
+
+// One row of the march: loads for g(y+1), f(y) and flags(y+2) are issued
+// first; then psi(y+1) (after bounce-back, walls and Zou-He on g), the LDS
+// exchange of the psi row, the stencil of row y, Zou-He on f, the moments and
+// the collision (MODE 0) or the field output (MODE 1).
+//   gc, flc  g and flags of row y (in);  gn  g of row y+1 (out)
+//   fln      flags of row y+1 (in, loaded one row ahead); fln2 row y+2 (out)
+//   pm, pc   psi of rows y-1 and y (in);  pn  psi of row y+1 (out)
+template <int NT, int MODE>
+__device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
+                                          float (*ring)[NT], int t, const Lane& ln, bool interior, int y, int r,
+                                          const float gc[9], unsigned flc, unsigned fln, unsigned& fln2, float pm,
+                                          float pc, float gn[9], float& pn, double& s_rho, double& s_psi,
+                                          double& s_bad) {
+    const int yn = y + 1;
+    const bool ghost_row = k.y_walls && yn >= k.H;   // psi_wall ghost row (fingering.py:224)
+    const int ynw = (yn >= k.H) ? yn - k.H : yn;
+    // inert solid nodes need neither their own pulls nor psi from g
+    const bool skip_n = (fln & kSolid) && k.inert;
+    const bool skip_c = (flc & kSolid) && k.inert;
+    const bool pull_n = ln.inside && !ghost_row;
+    float f[9];
+    const Rows rn = rows_of(k, ynw);
+    const Rows rc = rows_of(k, y);
+    if (pull_n) pull_raw(src, k, 9, rn, skip_n ? kOOB : ln.x * 4u, gn);
+    if (interior) pull_raw(src, k, 0, rc, skip_c ? kOOB : ln.x * 4u, f);
+    fln2 = flags_of(k, b, ln, y + 2);
+    if (pull_n) {
+        pull_fix(src, k, 9, rn, ynw, ln.x, fln, gn);
+        bc_g(k, ynw, (int)ln.x, gn);
+        pn = (fln & kSolid) ? k.c.psi_wall
+                            : gn[0] + gn[1] + gn[2] + gn[3] + gn[4] + gn[5] + gn[6] + gn[7] + gn[8];
+    } else {
+        pn = ghost_row ? k.c.psi_wall : ln.ghost;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gn[i] = 0.f;
+    }
+    ring[(r + 2) & 3][t] = pn;
+    __syncthreads();
+    if (!interior) return;
+    const float* Rm = ring[r & 3];
+    const float* Rc = ring[(r + 1) & 3];
+    const float* Rp = ring[(r + 2) & 3];
+    const float n[9] = {pc, Rc[t + 1], pn, Rc[t - 1], pm, Rp[t + 1], Rp[t - 1], Rm[t - 1], Rm[t + 1]};
+    float dpx, dpy, lap;
+    fdn::stencil(n, dpx, dpy, lap);
+    pull_fix(src, k, 0, rc, y, ln.x, flc, f);
+    bc_f(k, b, src, y, (int)ln.x, f);
+    float g[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = gc[i];
+    const unsigned o = rc.c + ln.x;
+    const bool solid = (flc & kSolid) != 0;
+    if (MODE == 0) {
+        if (!k.x_periodic && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
+            const float mu = fdn::chem_pot(pc, lap, k.c);
+            const int s = (ln.x == 0) ? 0 : 2;
+            b.side_out[s * k.H + y] = dpx * mu;
+            b.side_out[(s + 1) * k.H + y] = dpy * mu;
+        }
+        if (!solid) {
+            const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
+            fdn::collide(f, g, m, k.c);
+        }
+        const unsigned xb = skip_c ? kOOB : ln.x * 4u;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            bst(f[i], dst, xb, i * k.Q + rc.c);
+            bst(g[i], dst, xb, (9 + i) * k.Q + rc.c);
+        }
+        if (k.inert) slot_bounce(k, dst, y, ln.x, flc, f, g);
+    } else {
+        float rho = 0.f, ux = 0.f, uy = 0.f;
+        if (!solid) {
+            const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
+            rho = m.rho;
+            ux = m.ux;
+            uy = m.uy;
+            s_rho += (double)rho;
+            s_psi += (double)pc;
+            if (!isfinite(rho) || !isfinite(pc)) s_bad += 1.0;
+        }
+        b.fields[o] = pc;
+        b.fields[k.Q + o] = rho;
+        b.fields[2 * k.Q + o] = ux;
+        b.fields[3 * k.Q + o] = uy;
+        if (b.dst) {
+            // inert solid nodes carry no populations; report zeros there
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                b.dst[i * k.Q + o] = skip_c ? 0.f : f[i];
+                b.dst[(9 + i) * k.Q + o] = skip_c ? 0.f : g[i];
+            }
+        }
+    }
+}
+
+// psi and g of row y for this lane (the march prologue); fl = flags_of(row y).
+__device__ __forceinline__ float psi_row(const KParams& k, rsrc_t src, const Lane& ln, int y, unsigned fl,
+                                         float g[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = 0.f;
+    if (k.y_walls && (y < 0 || y >= k.H)) return k.c.psi_wall;
+    if (!ln.inside) return ln.ghost;
+    if ((fl & kSolid) && k.inert) return k.c.psi_wall;
+    y = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
+    const Rows r = rows_of(k, y);
+    pull_raw(src, k, 9, r, ln.x * 4u, g);
+    pull_fix(src, k, 9, r, y, ln.x, fl, g);
+    bc_g(k, y, (int)ln.x, g);
+    if (fl & kSolid) return k.c.psi_wall;
+    return g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+}
+
 // MODE 0: one iteration (collide and store the next post-collision state).
 // MODE 1: moments only - fields psi/rho/ux/uy, FP64 partial sums, and (if
 //         b.dst) the post-stream populations; the state is not advanced.
+// A workgroup of NT = 64*WX lanes owns the strip of columns
+// [strip*(NT-2), strip*(NT-2) + NT-2) and rows [chunk*rows, ...); lanes 0
+// and NT-1 only supply psi of the columns beside the strip.
 template <int WX, int MODE>
 __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     constexpr int NT = 64 * WX;
@@ -208,83 +457,42 @@ __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     const int t = threadIdx.x;
     const int strip = blockIdx.x % k.nstrips;
     const int chunk = blockIdx.x / k.nstrips;
-    const int x = strip * SW - 1 + t;
-    const bool interior = (t >= 1) && (t <= SW) && (x < k.W);
+    const int xcol = strip * SW - 1 + t;
+    const bool interior = (t >= 1) && (t <= SW) && (xcol < k.W);
+    const Lane ln = lane_of(k, xcol);
     const int y0 = chunk * k.rows;
     const int y1 = min(y0 + k.rows, k.H);
 
-    float gc[9], gp[9];
-    unsigned flc, flp;
+    const unsigned bytes = 18u * (unsigned)k.Q * 4u;
+    const rsrc_t src = make_rsrc(b.src, bytes);
+    const rsrc_t dst = make_rsrc(b.dst, b.dst ? bytes : 0u);
+    float gA[9], gB[9];
     double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
-
-    float psi_m = psi_node(k, b, y0 - 1, x, gp, flp);
-    float psi_c = psi_node(k, b, y0, x, gc, flc);
-    ring[0][t] = psi_m;
-    ring[1][t] = psi_c;
-
-    for (int y = y0; y < y1; ++y) {
-        const int r = y - y0;
-        const float psi_p = psi_node(k, b, y + 1, x, gp, flp);
-        ring[(r + 2) & 3][t] = psi_p;
-        __syncthreads();
-        if (interior) {
-            const float* Rm = ring[r & 3];
-            const float* Rc = ring[(r + 1) & 3];
-            const float* Rp = ring[(r + 2) & 3];
-            const float n[9] = {psi_c,     Rc[t + 1], psi_p,     Rc[t - 1], psi_m,
-                                Rp[t + 1], Rp[t - 1], Rm[t - 1], Rm[t + 1]};
-            float dpx, dpy, lap;
-            fdn::stencil(n, dpx, dpy, lap);
-            float f[9];
-            pull(b.src, k, 0, y, x, flc, f);
-            bc_f(k, b, y, x, f);
-            const int o = y * k.Wp + x;
-            const bool solid = (flc & kSolid) != 0;
-            if (MODE == 0) {
-                if (!k.x_periodic && (x == 0 || x == k.W - 1)) {
-                    const float mu = fdn::chem_pot(psi_c, lap, k.c);
-                    const int s = (x == 0) ? 0 : 2;
-                    b.side_out[s * k.H + y] = dpx * mu;
-                    b.side_out[(s + 1) * k.H + y] = dpy * mu;
-                }
-                if (!solid) {
-                    const fdn::Mom<float> m = fdn::moments(f, psi_c, dpx, dpy, lap, k.c);
-                    fdn::collide(f, gc, m, k.c);
-                }
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    b.dst[i * k.Q + o] = f[i];
-                    b.dst[(9 + i) * k.Q + o] = gc[i];
-                }
-            } else {
-                float rho = 0.f, ux = 0.f, uy = 0.f;
-                if (!solid) {
-                    const fdn::Mom<float> m = fdn::moments(f, psi_c, dpx, dpy, lap, k.c);
-                    rho = m.rho;
-                    ux = m.ux;
-                    uy = m.uy;
-                    s_rho += (double)rho;
-                    s_psi += (double)psi_c;
-                    if (!isfinite(rho) || !isfinite(psi_c)) s_bad += 1.0;
-                }
-                b.fields[o] = psi_c;
-                b.fields[k.Q + o] = rho;
-                b.fields[2 * k.Q + o] = ux;
-                b.fields[3 * k.Q + o] = uy;
-                if (b.dst) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) {
-                        b.dst[i * k.Q + o] = f[i];
-                        b.dst[(9 + i) * k.Q + o] = gc[i];
-                    }
-                }
-            }
+    // rolling flags: fA row y, fB row y+1, fC row y+2 (loaded one row ahead)
+    unsigned fA = flags_of(k, b, ln, y0);
+    unsigned fB = flags_of(k, b, ln, y0 + 1);
+    unsigned fC;
+    float pm = psi_row(k, src, ln, y0 - 1, flags_of(k, b, ln, y0 - 1), gB);
+    float pc = psi_row(k, src, ln, y0, fA, gA);
+    float pn;
+    ring[0][t] = pm;
+    ring[1][t] = pc;
+    // two rows per trip so the rolling g / psi / flag rows swap roles statically
+    for (int y = y0; y < y1; y += 2) {
+        march_row<NT, MODE>(k, b, src, dst, ring, t, ln, interior, y, y - y0, gA, fA, fB, fC, pm, pc, gB, pn,
+                            s_rho, s_psi, s_bad);
+        pm = pc;
+        pc = pn;
+        if (y + 1 < y1) {
+            march_row<NT, MODE>(k, b, src, dst, ring, t, ln, interior, y + 1, y + 1 - y0, gB, fB, fC, fA, pm, pc,
+                                gA, pn, s_rho, s_psi, s_bad);
+            pm = pc;
+            pc = pn;
+            // the second row wrote flags of row y+3 into fA: rows y+2, y+3 -> fA, fB
+            const unsigned f3 = fA;
+            fA = fC;
+            fB = f3;
         }
-        psi_m = psi_c;
-        psi_c = psi_p;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) gc[i] = gp[i];
-        flc = flp;
     }
     if (MODE == 1) block_sum3<NT>(s_rho, s_psi, s_bad, b.partials + 3 * blockIdx.x);
 }
@@ -343,6 +551,18 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             b.dst[i * k.Q + o] = f[i];
             b.dst[(9 + i) * k.Q + o] = g[i];
         }
+        if (!PRE && k.inert && !solid) {
+            // bounce-back through the solid neighbours' slots, as the step kernel does
+            const unsigned links = b.flags[o] & kSlotLinks;
+#pragma unroll
+            for (int i = 1; i < 9; ++i) {
+                if (!((links >> (i - 1)) & 1u)) continue;
+                const int ys = wrap(y - fdn::EY[i], k.H);
+                const int xs = wrap(x - fdn::EX[i], k.W);
+                b.dst[i * k.Q + ys * k.Wp + xs] = f[fdn::OPP[i]];
+                b.dst[(9 + i) * k.Q + ys * k.Wp + xs] = g[fdn::OPP[i]];
+            }
+        }
         if (PRE) {
             b.fields[o] = psi;
             b.fields[k.Q + o] = solid ? 0.f : m.rho;
@@ -398,7 +618,8 @@ struct fd_handle {
     int Wp = 0;
     size_t Q = 0;
     float* pop[2] = {nullptr, nullptr};
-    uint16_t* flags = nullptr;
+    uint32_t* flags = nullptr;
+    bool inert = false;
     float* side[2] = {nullptr, nullptr};
     float* uprof = nullptr;
     float* psi0 = nullptr;
@@ -423,8 +644,10 @@ namespace {
 int check_params(const fd_params* p) {
     if (!p) return fail(FD_E_ARG, "params is NULL");
     if (p->H < 3 || p->W < 3) return fail(FD_E_ARG, "H and W must be >= 3");
-    if ((int64_t)p->H * (((int64_t)p->W + 63) / 64 * 64) * 18 >= (int64_t)1 << 31)
-        return fail(FD_E_ARG, "domain too large for 32-bit plane offsets");
+    // buffer descriptors address the 18-plane set with byte offsets below 2^31
+    // (the out-of-range offset used for masked lanes sits above that)
+    if ((int64_t)p->H * (((int64_t)p->W + 63) / 64 * 64) * 18 * 4 >= ((int64_t)1 << 31) - 4096)
+        return fail(FD_E_ARG, "domain too large: H * (W rounded up to 64) must stay below 29.8M nodes");
     if (p->y_bc != FD_Y_PERIODIC && p->y_bc != FD_Y_WALLS) return fail(FD_E_ARG, "bad y_bc");
     if (p->x_bc != FD_X_ZOUHE && p->x_bc != FD_X_PERIODIC) return fail(FD_E_ARG, "bad x_bc");
     if (p->profile != FD_PROFILE_GAUSS && p->profile != FD_PROFILE_CONST)
@@ -444,6 +667,73 @@ int check_params(const fd_params* p) {
     return FD_OK;
 }
 
+// Direction i of node (y, x) is rewritten after the pull by a boundary
+// condition (Zou-He columns, walls, walled inlet/outlet corners), so the
+// pulled value never matters there.
+bool overwritten(const fd_params& p, int y, int x, int i) {
+    if (p.x_bc == FD_X_ZOUHE) {
+        if (x == 0 && (i == 1 || i == 5 || i == 8)) return true;
+        if (x == p.W - 1 && (i == 3 || i == 6 || i == 7)) return true;
+        if (p.y_bc == FD_Y_WALLS) {
+            if (x == 0 && y == 0 && (i == 2 || i == 6)) return true;
+            if (x == 0 && y == p.H - 1 && (i == 4 || i == 7)) return true;
+            if (x == p.W - 1 && y == 0 && i == 2) return true;
+            if (x == p.W - 1 && y == p.H - 1 && i == 4) return true;
+        }
+    }
+    if (p.y_bc == FD_Y_WALLS) {
+        if (y == p.wall_lo && (i == 2 || i == 5 || i == 6)) return true;
+        if (y == p.wall_hi && (i == 4 || i == 7 || i == 8)) return true;
+    }
+    return false;
+}
+
+// Pack the node flags and decide whether solid nodes are inert: that needs
+// every solid -> fluid link either bounced or rewritten by a boundary
+// condition (no leak of a solid node's streamed populations) and no solid
+// node on a Zou-He column.  Then links whose upstream node is solid become
+// slot links and the rest (over-bounced links, bounce_back.py:62-86) stay
+// explicit; otherwise every link is explicit.
+bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, int Wp, std::vector<uint32_t>& fl) {
+    const int H = p.H, W = p.W;
+    auto sol = [&](int y, int x) { return solid[(size_t)((y + H) % H) * W + ((x + W) % W)] != 0; };
+    bool inert = true;
+    for (int y = 0; y < H && inert; ++y) {
+        if (p.x_bc == FD_X_ZOUHE && (sol(y, 0) || sol(y, W - 1))) inert = false;
+        for (int x = 0; x < W && inert; ++x) {
+            if (sol(y, x)) continue;
+            const unsigned lk = links[(size_t)y * W + x];
+            for (int i = 1; i < 9; ++i) {
+                const bool up = sol(y - fdn::EY[i], x - fdn::EX[i]);
+                if (up && !((lk >> (i - 1)) & 1u) && !overwritten(p, y, x, i)) {
+                    inert = false;
+                    break;
+                }
+            }
+        }
+    }
+    fl.assign((size_t)H * Wp, kSolid);  // padding columns: solid, never read
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const unsigned lk = links[(size_t)y * W + x] & FD_LINK_BITS;
+            const bool s = sol(y, x);
+            unsigned v = s ? kSolid : 0u;
+            if (!inert) {
+                v |= lk << kExplShift;
+            } else if (!s) {
+                for (int i = 1; i < 9; ++i) {
+                    if (!((lk >> (i - 1)) & 1u)) continue;
+                    if (sol(y - fdn::EY[i], x - fdn::EX[i]))
+                        v |= 1u << (i - 1);
+                    else
+                        v |= 1u << (kExplShift + i - 1);
+                }
+            }
+            fl[(size_t)y * Wp + x] = v;
+        }
+    return inert;
+}
+
 void fill_kparams(fd_handle* h) {
     const fd_params& p = h->p;
     KParams& k = h->k;
@@ -456,6 +746,7 @@ void fill_kparams(fd_handle* h) {
     k.pressure = p.outlet == FD_OUTLET_PRESSURE;
     k.wall_lo = p.wall_lo;
     k.wall_hi = p.wall_hi;
+    k.inert = h->inert ? 1 : 0;
     k.rho_out = (float)p.rho_out;
     k.c3 = (float)p.outlet_f3_coef;
     k.c.inv_tau_g = (float)(1.0 / p.tau);
@@ -596,16 +887,14 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     h->device = dev;
     h->Wp = (p->W + 63) / 64 * 64;
     h->Q = (size_t)p->H * h->Wp;
+    std::vector<uint32_t> fl;
+    h->inert = pack_flags(*p, solid, links, h->Wp, fl);
     fill_kparams(h);
     // A large domain favours 4-wave strips; small ones need more workgroups.
     h->wx = p->W >= 1024 ? 4 : (p->W >= 256 ? 2 : 1);
     h->rows = p->H >= 1024 ? 32 : 16;
 
-    const size_t Q = h->Q, H = p->H, W = p->W;
-    std::vector<uint16_t> fl(Q, (uint16_t)kSolid);  // padding columns: solid, never read
-    for (size_t y = 0; y < H; ++y)
-        for (size_t x = 0; x < W; ++x)
-            fl[y * h->Wp + x] = (uint16_t)((links[y * W + x] & FD_LINK_BITS) | (solid[y * W + x] ? kSolid : 0u));
+    const size_t Q = h->Q, H = p->H;
     std::vector<float> up(H);
     for (size_t y = 0; y < H; ++y) {
         if (p->profile == FD_PROFILE_GAUSS) {
@@ -635,13 +924,13 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
         FD_HIP_H(hipMemsetAsync(h->pop[i], 0, sizeof(float) * 18 * Q, h->stream));
         FD_HIP_H(hipMemsetAsync(h->side[i], 0, sizeof(float) * 4 * H, h->stream));
     }
-    FD_HIP_H(hipMalloc(&h->flags, sizeof(uint16_t) * Q));
+    FD_HIP_H(hipMalloc(&h->flags, sizeof(uint32_t) * Q));
     FD_HIP_H(hipMalloc(&h->uprof, sizeof(float) * H));
     FD_HIP_H(hipMalloc(&h->psi0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->rho0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->fields, sizeof(float) * 4 * Q));
     FD_HIP_H(hipMalloc(&h->mass, sizeof(double) * 3));
-    FD_HIP_H(hipMemcpyAsync(h->flags, fl.data(), sizeof(uint16_t) * Q, hipMemcpyHostToDevice, h->stream));
+    FD_HIP_H(hipMemcpyAsync(h->flags, fl.data(), sizeof(uint32_t) * Q, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipMemcpyAsync(h->uprof, up.data(), sizeof(float) * H, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipStreamSynchronize(h->stream));
 #undef FD_HIP_H
@@ -714,6 +1003,8 @@ int fd_step(fd_handle* h, int32_t nsteps) {
 
 int64_t fd_steps_done(const fd_handle* h) { return h ? h->N : -1; }
 
+int fd_solid_inert(const fd_handle* h) { return h ? (h->inert ? 1 : 0) : -1; }
+
 int fd_synchronize(fd_handle* h) {
     if (!h) return fail(FD_E_ARG, "handle is NULL");
     FD_HIP(hipStreamSynchronize(h->stream));
@@ -728,8 +1019,11 @@ double fd_last_step_ms(const fd_handle* h) {
     return (double)ms / h->timed_launches;
 }
 
-int fd_set_rows_per_wave(fd_handle* h, int32_t rows) {
-    if (!h || rows < 1) return fail(FD_E_ARG, "rows must be >= 1");
+int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    if (waves_x != 1 && waves_x != 2 && waves_x != 4) return fail(FD_E_ARG, "waves_x must be 1, 2 or 4");
+    if (rows < 1) return fail(FD_E_ARG, "rows must be >= 1");
+    h->wx = waves_x;
     h->rows = rows;
     return FD_OK;
 }

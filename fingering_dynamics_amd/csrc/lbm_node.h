@@ -29,6 +29,17 @@
 
 namespace fdn {
 
+// Reciprocal: the hardware approximation (v_rcp_f32, 1 ulp) in FP32 device
+// code, an exact division elsewhere.
+FD_HD float rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+FD_HD double rcp(double x) { return 1.0 / x; }
+
 // D2Q9 order of fingering_periodic.py:50-89.
 constexpr int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -70,8 +81,8 @@ template <typename T>
 FD_HD T inv_tau_mix(T rho, T psi, const Consts<T>& c) {
     // 2 nu1 nu2 / (nu1 (1-psi) + nu2 (1+psi)) with nu1 = eta/rho, nu2 = M eta/rho
     // = 2 M eta / (rho ((1-psi) + M (1+psi)))
-    const T tau = T(6) * c.M * c.eta_n / (rho * ((T(1) - psi) + c.M * (T(1) + psi))) + T(0.5);
-    return T(1) / tau;
+    const T tau = T(6) * c.M * c.eta_n * rcp(rho * ((T(1) - psi) + c.M * (T(1) + psi))) + T(0.5);
+    return rcp(tau);
 }
 
 // Moments the collision of one node uses.
@@ -91,7 +102,7 @@ FD_HD Mom<T> moments(const T f[9], T psi, T dpx, T dpy, T lap, const Consts<T>& 
     m.mu = chem_pot(psi, lap, c);
     m.fx = m.mu * dpx + c.gx;
     m.fy = m.mu * dpy + c.gy;
-    const T ir = T(1) / m.rho;
+    const T ir = rcp(m.rho);
     m.ux = (jx + T(0.5) * m.fx) * ir;
     m.uy = (jy + T(0.5) * m.fy) * ir;
     m.p = m.rho * (T(1) / T(3)) + psi * m.mu;
@@ -104,8 +115,9 @@ FD_HD void equilibria(const Mom<T>& m, const Consts<T>& c, T feq[9], T geq[9]) {
     const T usq = m.ux * m.ux + m.uy * m.uy;
     const T A = T(3) * m.p;
     const T B = T(3) * c.gamma * m.mu;
-    const T A0 = (m.rho - T(3) * (T(1) - W9<T>::w0) * m.p) / W9<T>::w0;
-    const T B0 = (m.psi - T(3) * (T(1) - W9<T>::w0) * c.gamma * m.mu) / W9<T>::w0;
+    // (x - 3 (1 - w0) y) / w0 with w0 = 4/9: (x - 5/3 y) * 9/4
+    const T A0 = (m.rho - T(5) / T(3) * m.p) * T(2.25);
+    const T B0 = (m.psi - T(5) / T(3) * c.gamma * m.mu) * T(2.25);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T eu = T(EX[i]) * m.ux + T(EY[i]) * m.uy;
@@ -138,7 +150,7 @@ FD_HD void collide(T f[9], T g[9], const Mom<T>& m, const Consts<T>& c) {
 // Inlet density from the known populations (fingering_periodic.py:276-277).
 template <typename T>
 FD_HD T inlet_rho(const T f[9], T u, T px) {
-    return (f[0] + f[2] + f[4] + T(2) * (f[3] + f[6] + f[7]) - T(0.5) * px) / (T(1) - u);
+    return (f[0] + f[2] + f[4] + T(2) * (f[3] + f[6] + f[7]) - T(0.5) * px) * rcp(T(1) - u);
 }
 
 // Inlet, rows with the standard formulas (fingering_periodic.py:276-302).
@@ -210,9 +222,9 @@ FD_HD void outlet_f(T f[9], T u, T px, T py, bool pressure, T rho_out, T c3) {
     T r;
     if (pressure) {
         r = rho_out;
-        u = known / rho_out - T(1);
+        u = known * rcp(rho_out) - T(1);
     } else {
-        r = known / (T(1) + u);
+        r = known * rcp(T(1) + u);
     }
     const T ur = u * r;
     f[3] = f[1] - c3 * ur + px * (T(1) / T(6));

@@ -290,6 +290,29 @@ class Bounce_back:
         return bits
 
 
+def link_bits_complete(mask):
+    """Build-original: the complete half-way bounce-back set.  Bit (i-1) is
+    set at every fluid node whose upstream neighbour x - e_i (periodic
+    indexing, as in the streaming step) is solid.
+
+    The reference's circle classifier (create_block.py:81-112) leaves some
+    diagonal solid -> fluid links out of its masks; there the solid node's
+    streamed, never-collided populations (copies of what bounced off it)
+    re-enter the fluid and add mass.  With this set no population ever
+    leaves a solid node into fluid, so solid nodes need no update at all.
+    For the rectangles of halfway_bounceback_rec it differs from the
+    reference only by the reference's extra corner-adjacent bounces
+    (bounce_back.py:62-86)."""
+    fluid = np.asarray(mask, dtype=bool)
+    solid = ~fluid
+    bits = np.zeros(fluid.shape, dtype=np.uint8)
+    E = ((1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1))
+    for i, (ex, ey) in enumerate(E, start=1):
+        upstream_solid = np.roll(np.roll(solid, ey, axis=0), ex, axis=1)
+        bits[fluid & upstream_solid] |= np.uint8(1 << (i - 1))
+    return bits
+
+
 def unpack_links(bits):
     """uint8 bits [H][W] -> bool[9][H][W] per-direction masks (index 0 False)."""
     bits = np.asarray(bits, dtype=np.uint8)

@@ -86,6 +86,11 @@ class Compute:
         self._cache = None
 
     @property
+    def solid_inert(self):
+        """True when solid nodes are neither read nor written (leak-free geometry)."""
+        return bool(self._lib.fd_solid_inert(self._h))
+
+    @property
     def t(self):
         return int(self._lib.fd_steps_done(self._h))
 
@@ -95,8 +100,9 @@ class Compute:
     def last_step_ms(self):
         return float(self._lib.fd_last_step_ms(self._h))
 
-    def set_rows_per_wave(self, rows):
-        _native.check(self._lib.fd_set_rows_per_wave(self._h, int(rows)))
+    def set_tile(self, waves_x, rows):
+        """Launch shape: strips of 64*waves_x-2 columns, `rows` rows per workgroup."""
+        _native.check(self._lib.fd_set_tile(self._h, int(waves_x), int(rows)))
 
     # ------------------------------------------------------------ outputs
     def fields(self):

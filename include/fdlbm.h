@@ -139,9 +139,20 @@ int fd_synchronize(fd_handle* h);
  * with HIP events on the handle's stream; < 0 if unavailable. */
 double fd_last_step_ms(const fd_handle* h);
 
-/* Kernel launch shape: rows per wave-strip (>= 1).  Used by the tile
- * autotuner; the default suits large domains. */
-int fd_set_rows_per_wave(fd_handle* h, int32_t rows);
+/* Kernel launch shape: each workgroup owns a strip of 64*waves_x - 2
+ * columns (waves_x in {1, 2, 4}) and walks `rows` rows (>= 1).  Used by
+ * the tile autotuner; fd_create picks a default from the domain size.  No
+ * reference counterpart (a launch-shape control of this implementation). */
+int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
+
+/* 1 if the handle runs with inert solid nodes, 0 if solid nodes stream
+ * their populations on (store-and-forward, fingering_periodic.py:455-466),
+ * -1 for NULL.  Inert needs a leak-free geometry: every solid -> fluid link
+ * bounced (or rewritten by a boundary condition) and no solid node on a
+ * Zou-He column.  Solid nodes are then neither read nor written, bounced
+ * links cost no extra load, and populations reported on solid nodes are 0.
+ * No reference counterpart (a property of this implementation). */
+int fd_solid_inert(const fd_handle* h);
 
 /* Release every device buffer of the handle.  NULL is ignored. */
 void fd_destroy(fd_handle* h);

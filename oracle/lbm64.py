@@ -453,6 +453,24 @@ class Lbm64:
             self.moments()
             self.t += 1
 
+    @classmethod
+    def from_params(cls, p, mask, links, psi0=None, rho0=None):
+        """Build from any object with the attributes of the product's
+        parameter record (H, W, tau, eta_n, M, a, kappa, gamma, psi_wall,
+        u0, rho0, rho_out, outlet_f3_coef, gx, gy, y_bc, x_bc, profile,
+        outlet, init_mode, wall_lo, wall_hi, n_inv_cols) using the integer
+        codes of include/fdlbm.h."""
+        wall_hi = p.H - 1 if p.wall_hi < 0 else p.wall_hi
+        return cls(
+            mask, links, tau=p.tau, eta_n=p.eta_n, M=p.M, kappa=p.kappa, a=p.a, gamma=p.gamma,
+            u0=p.u0, psi_wall=p.psi_wall, rho0=p.rho0, n_inv_cols=p.n_inv_cols,
+            inlet="const" if p.profile == 1 else "gauss",
+            outlet="pressure" if p.outlet == 1 else "velocity",
+            outlet_f3_coef=p.outlet_f3_coef, y_walls=p.y_bc == 1, x_periodic=p.x_bc == 1,
+            init="walled" if p.init_mode == 1 else "mu0", body_force=(p.gx, p.gy),
+            rho_out=p.rho_out, psi0=psi0, rho_init=rho0,
+            wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped")
+
     # ---------------------------------------------------------- outputs
     def field(self, name):
         """Full [H][W] view of a compressed moment (0 on solid nodes)."""

@@ -3,8 +3,7 @@ the GPU solver (fingering_dynamics_amd) and the FP64 oracle."""
 import numpy as np
 
 from fingering_dynamics_amd import geometry
-from fingering_dynamics_amd.params import (LbmParams, Y_WALLS, X_PERIODIC, PROFILE_CONST,
-                                           OUTLET_PRESSURE, INIT_WALLED)
+from fingering_dynamics_amd.params import LbmParams
 from oracle import geometry64, lbm64
 
 
@@ -34,16 +33,7 @@ def circle_domain(H, W, r_min, r_max, porosity, seed=2, x_lo=None):
 
 def oracle_for(params: LbmParams, mask, links, psi0=None, rho0=None):
     """An Lbm64 configured exactly like ``params``."""
-    p = params
-    return lbm64.Lbm64(
-        mask, links, tau=p.tau, eta_n=p.eta_n, M=p.M, kappa=p.kappa, a=p.a, gamma=p.gamma,
-        u0=p.u0, psi_wall=p.psi_wall, rho0=p.rho0, n_inv_cols=p.n_inv_cols,
-        inlet="const" if p.profile == PROFILE_CONST else "gauss",
-        outlet="pressure" if p.outlet == OUTLET_PRESSURE else "velocity",
-        outlet_f3_coef=p.outlet_f3_coef, y_walls=p.y_bc == Y_WALLS,
-        x_periodic=p.x_bc == X_PERIODIC, init="walled" if p.init_mode == INIT_WALLED else "mu0",
-        body_force=(p.gx, p.gy), rho_out=p.rho_out, psi0=psi0, rho_init=rho0,
-        wall_rows=("edge" if (p.wall_lo == 0 and p.wall_hi_row == p.H - 1) else "shipped"))
+    return lbm64.Lbm64.from_params(params, mask, links, psi0=psi0, rho0=rho0)
 
 
 def oracle_fields(o):

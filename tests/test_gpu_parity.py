@@ -26,14 +26,19 @@ def _compute(*a, **k):
 
 
 def _compare(cm, o, mask, tag, pops=True):
+    """Fields on every node; populations on fluid nodes, and on solid nodes
+    too when they stream their populations on (store-and-forward)."""
     psi, rho, ux, uy = cm.fields()
     opsi, orho, oux, ouy = oracle_fields(o)
     e = dict(psi=max_err(psi, opsi), rho=max_err(rho, orho, mask),
              ux=max_err(ux, oux, mask), uy=max_err(uy, ouy, mask))
     if pops:
         f, g = cm.populations()
-        e["f"] = max_err(f, o.f)
-        e["g"] = max_err(g, o.g)
+        sel = mask if cm.solid_inert else np.ones_like(mask)
+        e["f"] = max_err(f[:, sel], o.f[:, sel])
+        e["g"] = max_err(g[:, sel], o.g[:, sel])
+        if cm.solid_inert:
+            assert not f[:, ~mask].any() and not g[:, ~mask].any()
     print(tag, {k: "%.2e" % v for k, v in e.items()}, flush=True)
     assert e["psi"] < TOL_POP and e["rho"] < TOL_POP, (tag, e)
     assert e["ux"] < TOL_U and e["uy"] < TOL_U, (tag, e)
@@ -98,6 +103,27 @@ def test_parity_small(case):
     _run_pair(params, mask, bits, links, [1, 2, 5, 20, 50])
 
 
+def test_inert_mode_selection():
+    """Rectangles (reference link sets) and complete link sets are
+    leak-free, so solid nodes are inert; the reference's circle classifier
+    leaves diagonal leaks, so there solid nodes store and forward."""
+    from fingering_dynamics_amd import geometry
+    params, mask, bits, links = _setup("periodic_cfg1")
+    assert _compute(mask, bits, params).solid_inert
+    params, mask, bits, links = _setup("periodic_circles")
+    assert not _compute(mask, bits, params).solid_inert
+    assert _compute(mask, geometry.link_bits_complete(mask), params).solid_inert
+
+
+def test_parity_complete_links_circles():
+    """Circles with the complete half-way link set: inert solids, every
+    bounce through the solid neighbour's slot."""
+    from fingering_dynamics_amd import geometry
+    params, mask, _, _ = _setup("periodic_circles")
+    bits = geometry.link_bits_complete(mask)
+    _run_pair(params, mask, bits, geometry.unpack_links(bits), [1, 2, 5, 20, 50])
+
+
 def test_parity_non_multiple_shapes():
     # W not a multiple of the strip width, odd row chunks, several strips
     for H, W in ((38, 131), (66, 300)):
@@ -113,13 +139,13 @@ def test_parity_rows_per_wave_sweep():
     o = oracle_for(params, mask, links)
     o.step(9)
     ref = oracle_fields(o)
-    for rows in (1, 3, 7, 16, 40, 64):
+    for wx, rows in ((1, 1), (1, 3), (2, 7), (4, 16), (1, 40), (2, 64), (4, 5)):
         cm = _compute(mask, bits, params)
-        cm.set_rows_per_wave(rows)
+        cm.set_tile(wx, rows)
         cm.step(9)
         psi, rho, ux, uy = cm.fields()
-        assert max_err(psi, ref[0]) < TOL_POP, rows
-        assert max_err(ux, ref[2], mask) < TOL_U, rows
+        assert max_err(psi, ref[0]) < TOL_POP, (wx, rows)
+        assert max_err(ux, ref[2], mask) < TOL_U, (wx, rows)
         cm.close()
 
 
