@@ -114,7 +114,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=15,
                     help="iterations of the CPU baseline sample (0 skips it)")
     ap.add_argument("--tile", default="", help="launch shape WX,ROWS (default: the library's choice)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
     args = ap.parse_args()
+    if args.hbm_probe:
+        import ctypes
+        from fingering_dynamics_amd import _native
+        lib = _native.load()
+        ms = ctypes.c_double()
+        nbytes = 1 << 30
+        _native.check(lib.fd_copy_probe(nbytes, 20, ctypes.byref(ms)))
+        print(json.dumps({"probe": "plain copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "ms_per_copy": ms.value, "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)

@@ -106,25 +106,36 @@ __device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vbyte, int sidx)
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vbyte, sidx * 4, 0);
 }
 
-// Streaming loads of one distribution (planes P0..P0+8) into node (y, x):
-// direction i comes from (y - e_iy, x - e_ix).  x - 1 and x + 1 are not
-// wrapped: at x = 0 and x = W-1 the affected directions (1, 5, 8 and 3, 6,
-// 7) are either rewritten by Zou-He or, for periodic x, reloaded by
-// pull_fix.  The -1 shifts only occur in planes >= 1, so every scalar index
-// is non-negative.
-__device__ __forceinline__ void pull_raw(rsrc_t src, const KParams& k, int P0, const Rows& r, unsigned xb,
+// Per-lane byte offsets of the nine pull sources inside the f planes:
+// direction i reads plane i at column x - e_ix, i.e. ((x - e_ix) + i*Q)*4.
+// They do not change along the march, so they live in vector registers and
+// each load only adds a row offset in a scalar register (g planes: + 9*Q*4).
+// x - 1 and x + 1 are not wrapped: at x = 0 and x = W-1 the affected
+// directions (1, 5, 8 and 3, 6, 7) are either rewritten by Zou-He or, for
+// periodic x, reloaded by pull_fix; the -1 shifts only occur in planes >= 1,
+// so every offset is non-negative and inside the allocation.
+struct PullOff {
+    unsigned v[9];
+};
+
+__device__ __forceinline__ PullOff pull_offsets(const KParams& k, unsigned x) {
+    PullOff po;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) po.v[i] = (unsigned)((int)x - fdn::EX[i] + i * k.Q) * 4u;
+    return po;
+}
+
+// Streaming loads of one distribution (plane base P0 = 0 for f, 9 for g)
+// into the node whose pull offsets are po, at the rows of r.
+__device__ __forceinline__ void pull_raw(rsrc_t src, const KParams& k, int P0, const Rows& r, const PullOff& po,
                                          float v[9]) {
-    const int b = P0 * k.Q;
-    const int Q = k.Q;
-    v[0] = bld(src, xb, b + r.c);
-    v[1] = bld(src, xb, b + Q + r.c - 1);
-    v[2] = bld(src, xb, b + 2 * Q + r.s);
-    v[3] = bld(src, xb, b + 3 * Q + r.c + 1);
-    v[4] = bld(src, xb, b + 4 * Q + r.n);
-    v[5] = bld(src, xb, b + 5 * Q + r.s - 1);
-    v[6] = bld(src, xb, b + 6 * Q + r.s + 1);
-    v[7] = bld(src, xb, b + 7 * Q + r.n + 1);
-    v[8] = bld(src, xb, b + 8 * Q + r.n - 1);
+    const int gq = P0 * k.Q;
+    const int rs = r.s + gq, rc = r.c + gq, rn = r.n + gq;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int row = fdn::EY[i] > 0 ? rs : (fdn::EY[i] < 0 ? rn : rc);
+        v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src, po.v[i], row * 4, 0));
+    }
 }
 
 // Periodic-x wrap for the first and last column, then the node's
@@ -180,7 +191,7 @@ __device__ __forceinline__ void pull_fix(rsrc_t src, const KParams& k, int P0, c
 __device__ __forceinline__ void pull(rsrc_t src, const KParams& k, int P0, int y, unsigned x, unsigned fl,
                                      float v[9]) {
     const Rows r = rows_of(k, y);
-    pull_raw(src, k, P0, r, x * 4u, v);
+    pull_raw(src, k, P0, r, pull_offsets(k, x), v);
     pull_fix(src, k, P0, r, y, x, fl, v);
 }
 
@@ -339,7 +350,8 @@ __device__ __forceinline__ void slot_bounce(const KParams& k, rsrc_t dst, int y,
 //   pm, pc   psi of rows y-1 and y (in);  pn  psi of row y+1 (out)
 template <int NT, int MODE>
 __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
-                                          float (*ring)[NT], int t, const Lane& ln, bool interior, int y, int r,
+                                          float (*ring)[NT], int t, const Lane& ln, const PullOff& po,
+                                          bool interior, int y, int r,
                                           const float gc[9], unsigned flc, unsigned fln, unsigned& fln2, float pm,
                                           float pc, float gn[9], float& pn, double& s_rho, double& s_psi,
                                           double& s_bad) {
@@ -351,10 +363,12 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
     const bool skip_c = (flc & kSolid) && k.inert;
     const bool pull_n = ln.inside && !ghost_row;
     float f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gn[i] = f[i] = 0.f;
     const Rows rn = rows_of(k, ynw);
     const Rows rc = rows_of(k, y);
-    if (pull_n) pull_raw(src, k, 9, rn, skip_n ? kOOB : ln.x * 4u, gn);
-    if (interior) pull_raw(src, k, 0, rc, skip_c ? kOOB : ln.x * 4u, f);
+    if (pull_n && !skip_n) pull_raw(src, k, 9, rn, po, gn);
+    if (interior && !skip_c) pull_raw(src, k, 0, rc, po, f);
     fln2 = flags_of(k, b, ln, y + 2);
     if (pull_n) {
         pull_fix(src, k, 9, rn, ynw, ln.x, fln, gn);
@@ -363,8 +377,6 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
                             : gn[0] + gn[1] + gn[2] + gn[3] + gn[4] + gn[5] + gn[6] + gn[7] + gn[8];
     } else {
         pn = ghost_row ? k.c.psi_wall : ln.ghost;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) gn[i] = 0.f;
     }
     ring[(r + 2) & 3][t] = pn;
     __syncthreads();
@@ -393,11 +405,15 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
             const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
             fdn::collide(f, g, m, k.c);
         }
-        const unsigned xb = skip_c ? kOOB : ln.x * 4u;
+        if (!skip_c) {
+            // node (y, x) of plane i: the pull offset of direction i moved back by e_ix
+            const int rowf = rc.c * 4, rowg = (rc.c + 9 * k.Q) * 4;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            bst(f[i], dst, xb, i * k.Q + rc.c);
-            bst(g[i], dst, xb, (9 + i) * k.Q + rc.c);
+            for (int i = 0; i < 9; ++i) {
+                const unsigned vo = po.v[i] + (unsigned)(fdn::EX[i] * 4);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, rowf, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo, rowg, 0);
+            }
         }
         if (k.inert) slot_bounce(k, dst, y, ln.x, flc, f, g);
     } else {
@@ -427,8 +443,8 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
 }
 
 // psi and g of row y for this lane (the march prologue); fl = flags_of(row y).
-__device__ __forceinline__ float psi_row(const KParams& k, rsrc_t src, const Lane& ln, int y, unsigned fl,
-                                         float g[9]) {
+__device__ __forceinline__ float psi_row(const KParams& k, rsrc_t src, const Lane& ln, const PullOff& po, int y,
+                                         unsigned fl, float g[9]) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) g[i] = 0.f;
     if (k.y_walls && (y < 0 || y >= k.H)) return k.c.psi_wall;
@@ -436,7 +452,7 @@ __device__ __forceinline__ float psi_row(const KParams& k, rsrc_t src, const Lan
     if ((fl & kSolid) && k.inert) return k.c.psi_wall;
     y = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
     const Rows r = rows_of(k, y);
-    pull_raw(src, k, 9, r, ln.x * 4u, g);
+    pull_raw(src, k, 9, r, po, g);
     pull_fix(src, k, 9, r, y, ln.x, fl, g);
     bc_g(k, y, (int)ln.x, g);
     if (fl & kSolid) return k.c.psi_wall;
@@ -472,20 +488,21 @@ __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     unsigned fA = flags_of(k, b, ln, y0);
     unsigned fB = flags_of(k, b, ln, y0 + 1);
     unsigned fC;
-    float pm = psi_row(k, src, ln, y0 - 1, flags_of(k, b, ln, y0 - 1), gB);
-    float pc = psi_row(k, src, ln, y0, fA, gA);
+    const PullOff po = pull_offsets(k, ln.x);
+    float pm = psi_row(k, src, ln, po, y0 - 1, flags_of(k, b, ln, y0 - 1), gB);
+    float pc = psi_row(k, src, ln, po, y0, fA, gA);
     float pn;
     ring[0][t] = pm;
     ring[1][t] = pc;
     // two rows per trip so the rolling g / psi / flag rows swap roles statically
     for (int y = y0; y < y1; y += 2) {
-        march_row<NT, MODE>(k, b, src, dst, ring, t, ln, interior, y, y - y0, gA, fA, fB, fC, pm, pc, gB, pn,
+        march_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, gA, fA, fB, fC, pm, pc, gB, pn,
                             s_rho, s_psi, s_bad);
         pm = pc;
         pc = pn;
         if (y + 1 < y1) {
-            march_row<NT, MODE>(k, b, src, dst, ring, t, ln, interior, y + 1, y + 1 - y0, gB, fB, fC, fA, pm, pc,
-                                gA, pn, s_rho, s_psi, s_bad);
+            march_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, interior, y + 1, y + 1 - y0, gB, fB, fC, fA, pm,
+                                pc, gA, pn, s_rho, s_psi, s_bad);
             pm = pc;
             pc = pn;
             // the second row wrote flags of row y+3 into fA: rows y+2, y+3 -> fA, fB
@@ -546,10 +563,14 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             fdn::equilibria(m, k.c, f, g);
             if (!PRE) fdn::collide(f, g, m, k.c);
         }
+        // An inert solid node's slots belong to its fluid neighbours (slot
+        // bounce-back below), so it must not write them.
+        if (PRE || !(solid && k.inert)) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            b.dst[i * k.Q + o] = f[i];
-            b.dst[(9 + i) * k.Q + o] = g[i];
+            for (int i = 0; i < 9; ++i) {
+                b.dst[i * k.Q + o] = f[i];
+                b.dst[(9 + i) * k.Q + o] = g[i];
+            }
         }
         if (!PRE && k.inert && !solid) {
             // bounce-back through the solid neighbours' slots, as the step kernel does
@@ -581,6 +602,13 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
         }
     }
     if (PRE) block_sum3<256>(s_rho, s_psi, s_bad, b.partials + 3 * (blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+// Plain streaming copy (16 B per lane per trip), the HBM yardstick.
+__global__ __launch_bounds__(256) void fd_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b,
+                                                      int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
 __global__ __launch_bounds__(256) void reduce_partials(const double* __restrict__ part, int n,
@@ -1004,6 +1032,48 @@ int fd_step(fd_handle* h, int32_t nsteps) {
 int64_t fd_steps_done(const fd_handle* h) { return h ? h->N : -1; }
 
 int fd_solid_inert(const fd_handle* h) { return h ? (h->inert ? 1 : 0) : -1; }
+
+int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy) {
+    if (nbytes < 16 || reps < 1 || !ms_per_copy) return fail(FD_E_ARG, "bad copy probe arguments");
+    const int64_t n = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FD_OK;
+    float ms = 0.f;
+#define FD_HIP_P(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            rc = fail(FD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+            goto done;                                                                     \
+        }                                                                                  \
+    } while (0)
+    FD_HIP_P(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    FD_HIP_P(hipEventCreate(&e0));
+    FD_HIP_P(hipEventCreate(&e1));
+    FD_HIP_P(hipMalloc(&a, n * 16));
+    FD_HIP_P(hipMalloc(&b, n * 16));
+    FD_HIP_P(hipMemsetAsync(a, 0, n * 16, st));
+    hipLaunchKernelGGL(fd_copy_kernel, dim3(8192), dim3(256), 0, st, a, b, n);  // warm-up
+    FD_HIP_P(hipGetLastError());
+    FD_HIP_P(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(fd_copy_kernel, dim3(8192), dim3(256), 0, st, a, b, n);
+    FD_HIP_P(hipGetLastError());
+    FD_HIP_P(hipEventRecord(e1, st));
+    FD_HIP_P(hipEventSynchronize(e1));
+    FD_HIP_P(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = (double)ms / reps;
+#undef FD_HIP_P
+done:
+    if (st) (void)hipStreamSynchronize(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
 
 int fd_synchronize(fd_handle* h) {
     if (!h) return fail(FD_E_ARG, "handle is NULL");

@@ -128,21 +128,51 @@ FD_HD void equilibria(const Mom<T>& m, const Consts<T>& c, T feq[9], T geq[9]) {
     }
 }
 
-// In-place BGK collision of both distributions with Guo forcing.
+// In-place BGK collision of both distributions with Guo forcing, written
+// over the direction pairs (i, opp(i)) that share |e.u| and |e.F|:
+//   f*_i = f_i - (f_i - feq_i)/tau_mix + F_i,  g*_i = g_i - (g_i - geq_i)/tau
+//   feq_i = w_i (A_i + rho Q_i),  geq_i = w_i (B_i + psi Q_i)
+//   Q_{i,opp} = S +- 3 eu,             S = 4.5 eu^2 - 1.5 u.u
+//   F_{i,opp} = w_i fac (T +- 3 eF),   T = 9 eu eF - 3 u.F
 template <typename T>
 FD_HD void collide(T f[9], T g[9], const Mom<T>& m, const Consts<T>& c) {
-    T feq[9], geq[9];
-    equilibria(m, c, feq, geq);
-    const T fac = T(1) - T(0.5) * m.itau;
+    const T usq = m.ux * m.ux + m.uy * m.uy;
+    const T uF = m.ux * m.fx + m.uy * m.fy;
+    const T it = m.itau, ig = c.inv_tau_g;
+    const T fac = T(1) - T(0.5) * it;
+    const T A = T(3) * m.p;
+    const T B = T(3) * c.gamma * m.mu;
+    // rest direction: Q0 = -1.5 u.u, F_0 = w0 fac (-3 u.F)
+    {
+        const T A0 = (m.rho - T(5) / T(3) * m.p) * T(2.25);
+        const T B0 = (m.psi - T(5) / T(3) * c.gamma * m.mu) * T(2.25);
+        const T q0 = T(-1.5) * usq;
+        const T w = W9<T>::w0;
+        const T feq = w * (A0 + m.rho * q0);
+        const T geq = w * (B0 + m.psi * q0);
+        f[0] = f[0] - (f[0] - feq) * it + w * fac * (T(-3) * uF);
+        g[0] = g[0] - (g[0] - geq) * ig;
+    }
+    // pairs (1,3) (2,4) (5,7) (6,8): e = (1,0) (0,1) (1,1) (-1,1)
+    constexpr int PI[4] = {1, 2, 5, 6};
+    constexpr int PO[4] = {3, 4, 7, 8};
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
+    for (int k = 0; k < 4; ++k) {
+        const int i = PI[k], o = PO[k];
         const T ex = T(EX[i]), ey = T(EY[i]);
+        const T w = W9<T>::w(i);
         const T eu = ex * m.ux + ey * m.uy;
-        const T Fi = W9<T>::w(i) * fac *
-                     ((T(3) * (ex - m.ux) + T(9) * ex * eu) * m.fx +
-                      (T(3) * (ey - m.uy) + T(9) * ey * eu) * m.fy);
-        f[i] = f[i] - (f[i] - feq[i]) * m.itau + Fi;
-        g[i] = g[i] - (g[i] - geq[i]) * c.inv_tau_g;
+        const T eF = ex * m.fx + ey * m.fy;
+        const T S = T(4.5) * eu * eu - T(1.5) * usq;
+        const T a3 = T(3) * eu;
+        const T Fs = w * fac * (T(9) * eu * eF - T(3) * uF);
+        const T Fa = w * fac * (T(3) * eF);
+        const T fp = w * (A + m.rho * (S + a3)), fm = w * (A + m.rho * (S - a3));
+        const T gp = w * (B + m.psi * (S + a3)), gm = w * (B + m.psi * (S - a3));
+        f[i] = f[i] - (f[i] - fp) * it + (Fs + Fa);
+        f[o] = f[o] - (f[o] - fm) * it + (Fs - Fa);
+        g[i] = g[i] - (g[i] - gp) * ig;
+        g[o] = g[o] - (g[o] - gm) * ig;
     }
 }
 

@@ -154,6 +154,13 @@ int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
  * No reference counterpart (a property of this implementation). */
 int fd_solid_inert(const fd_handle* h);
 
+/* HBM calibration: copies nbytes (rounded down to 16 B) between two device
+ * buffers the library allocates, `reps` times, with a plain streaming copy
+ * kernel (fd_copy_kernel), and returns the average milliseconds per copy.
+ * Used to price the roofline and to scale rocprofv3 FETCH_SIZE / WRITE_SIZE
+ * counters.  No reference counterpart (a measurement aid). */
+int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy);
+
 /* Release every device buffer of the handle.  NULL is ignored. */
 void fd_destroy(fd_handle* h);
 

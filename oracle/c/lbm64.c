@@ -19,10 +19,18 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Arithmetic and storage type: double for the oracle; built a second time
+   with -DO64_REAL=float as the FP32 evaluation of the same algorithm (the
+   yardstick for how far FP32 arithmetic itself moves the trajectory). */
+#ifndef O64_REAL
+#define O64_REAL double
+#endif
+typedef O64_REAL real_t;
+
 static const int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 static const int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 static const int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
-static const double WT[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+static const real_t WT[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
 
 typedef struct {
     int H, W;
@@ -39,9 +47,9 @@ typedef struct {
     o64_params p;
     int H, W, N;
     unsigned char *fluid, *links; /* links: [9][H][W] */
-    double *f, *g, *fb, *gb;      /* [9][H][W] */
-    double *psi, *rho, *ux, *uy, *mu, *p_, *tau_mix, *dx, *dy, *lap; /* [H][W] */
-    double *uprof;                /* [H] */
+    real_t *f, *g, *fb, *gb;      /* [9][H][W] */
+    real_t *psi, *rho, *ux, *uy, *mu, *p_, *tau_mix, *dx, *dy, *lap; /* [H][W] */
+    real_t *uprof;                /* [H] */
     long t;
 } o64;
 
@@ -51,7 +59,7 @@ typedef struct {
 static int wrapi(int v, int n) { return ((v % n) + n) % n; }
 
 /* psi with psi_wall on solids and the stencil's ghosts */
-static double psi_at(const o64* o, int y, int x) {
+static real_t psi_at(const o64* o, int y, int x) {
     if (o->p.y_walls && (y < 0 || y >= o->H)) return o->p.psi_wall;
     y = wrapi(y, o->H);
     if (!o->p.x_periodic) {
@@ -66,16 +74,16 @@ static void stencils(o64* o) { /* fingering_periodic.py:214-256 */
     /* psi padded by one ghost ring: psi_wall on solids, +1 / -1 ghost
        columns (or periodic x), psi_wall ghost rows (or periodic y) */
     const int H = o->H, W = o->W, P = W + 2;
-    double* t = malloc(sizeof(double) * (size_t)(H + 2) * P);
+    real_t* t = malloc(sizeof(real_t) * (size_t)(H + 2) * P);
     for (int y = -1; y <= H; ++y)
         for (int x = -1; x <= W; ++x) t[(size_t)(y + 1) * P + (x + 1)] = psi_at(o, y, x);
     for (int y = 0; y < H; ++y) {
-        const double* rs = t + (size_t)y * P + 1;
-        const double* rc = rs + P;
-        const double* rn = rc + P;
+        const real_t* rs = t + (size_t)y * P + 1;
+        const real_t* rc = rs + P;
+        const real_t* rn = rc + P;
         for (int x = 0; x < W; ++x) {
-            double C = rc[x], E = rc[x + 1], Wn = rc[x - 1], N = rn[x], S = rs[x];
-            double NE = rn[x + 1], NW = rn[x - 1], SW = rs[x - 1], SE = rs[x + 1];
+            real_t C = rc[x], E = rc[x + 1], Wn = rc[x - 1], N = rn[x], S = rs[x];
+            real_t NE = rn[x + 1], NW = rn[x - 1], SW = rs[x - 1], SE = rs[x + 1];
             o->dx[IDX(y, x)] = (4 * (E - Wn) + NE - NW - SW + SE) / 12;
             o->dy[IDX(y, x)] = (4 * (N - S) + NE + NW - SW - SE) / 12;
             o->lap[IDX(y, x)] = (-20 * C + 4 * (N + E + Wn + S) + NE + NW + SW + SE) / 6;
@@ -84,22 +92,22 @@ static void stencils(o64* o) { /* fingering_periodic.py:214-256 */
     free(t);
 }
 
-static double mu_plain(const o64* o, size_t n, double psi) { /* :146-149 */
+static real_t mu_plain(const o64* o, size_t n, real_t psi) { /* :146-149 */
     return o->p.a * psi * (1 - psi * psi) - o->p.kappa * o->lap[n];
 }
 
-static double mix_tau(const o64* o, double rho, double psi) { /* :201-208 */
-    double v1 = o->p.eta_n / rho, v2 = o->p.eta_n * o->p.M / rho;
+static real_t mix_tau(const o64* o, real_t rho, real_t psi) { /* :201-208 */
+    real_t v1 = o->p.eta_n / rho, v2 = o->p.eta_n * o->p.M / rho;
     return 3 * (2 * v1 * v2 / (v1 * (1 - psi) + v2 * (1 + psi))) + 0.5;
 }
 
-static void equilibria(const o64* o, size_t n, double feq[9], double geq[9]) { /* :155-192 */
-    double rho = o->rho[n], psi = o->psi[n], ux = o->ux[n], uy = o->uy[n], p = o->p_[n], mu = o->mu[n];
+static void equilibria(const o64* o, size_t n, real_t feq[9], real_t geq[9]) { /* :155-192 */
+    real_t rho = o->rho[n], psi = o->psi[n], ux = o->ux[n], uy = o->uy[n], p = o->p_[n], mu = o->mu[n];
     for (int i = 0; i < 9; ++i) {
-        double eu = EX[i] * ux + EY[i] * uy;
-        double q = 3 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy);
-        double A = i == 0 ? (rho - 3.0 * (1.0 - WT[0]) * p) / WT[0] : 3 * p;
-        double B = i == 0 ? (psi - 3.0 * (1.0 - WT[0]) * o->p.gamma * mu) / WT[0] : 3 * o->p.gamma * mu;
+        real_t eu = EX[i] * ux + EY[i] * uy;
+        real_t q = 3 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy);
+        real_t A = i == 0 ? (rho - 3.0 * (1.0 - WT[0]) * p) / WT[0] : 3 * p;
+        real_t B = i == 0 ? (psi - 3.0 * (1.0 - WT[0]) * o->p.gamma * mu) / WT[0] : 3 * o->p.gamma * mu;
         feq[i] = WT[i] * (A + rho * q);
         geq[i] = WT[i] * (B + psi * q);
     }
@@ -110,15 +118,15 @@ static void collide(o64* o) { /* :455-460, getLarge_F :194-199, getF/getG :258-2
         for (int x = 0; x < o->W; ++x) {
             size_t n = IDX(y, x);
             if (!o->fluid[n]) continue;
-            double feq[9], geq[9];
+            real_t feq[9], geq[9];
             equilibria(o, n, feq, geq);
-            double ux = o->ux[n], uy = o->uy[n], tm = o->tau_mix[n];
-            double Fx = o->mu[n] * o->dx[n] + o->p.gx, Fy = o->mu[n] * o->dy[n] + o->p.gy;
+            real_t ux = o->ux[n], uy = o->uy[n], tm = o->tau_mix[n];
+            real_t Fx = o->mu[n] * o->dx[n] + o->p.gx, Fy = o->mu[n] * o->dy[n] + o->p.gy;
             for (int i = 0; i < 9; ++i) {
-                double eu = EX[i] * ux + EY[i] * uy;
-                double F = WT[i] * (1 - 1 / (2 * tm)) *
+                real_t eu = EX[i] * ux + EY[i] * uy;
+                real_t F = WT[i] * (1 - 1 / (2 * tm)) *
                            (((EX[i] - ux) * 3 + EX[i] * eu * 9) * Fx + ((EY[i] - uy) * 3 + EY[i] * eu * 9) * Fy);
-                double fi = POP(o->f, i, y, x), gi = POP(o->g, i, y, x);
+                real_t fi = POP(o->f, i, y, x), gi = POP(o->g, i, y, x);
                 POP(o->f, i, y, x) = fi - 1.0 / tm * (fi - feq[i]) + F;
                 POP(o->g, i, y, x) = gi - 1.0 / o->p.tau * (gi - geq[i]);
             }
@@ -128,7 +136,7 @@ static void collide(o64* o) { /* :455-460, getLarge_F :194-199, getF/getG :258-2
 static void moments(o64* o) { /* :470-479 */
     for (size_t n = 0; n < (size_t)o->N; ++n)
         if (o->fluid[n]) {
-            double s = 0, ps = 0;
+            real_t s = 0, ps = 0;
             for (int i = 0; i < 9; ++i) {
                 s += o->f[(size_t)i * o->N + n];
                 ps += o->g[(size_t)i * o->N + n];
@@ -141,9 +149,9 @@ static void moments(o64* o) { /* :470-479 */
     stencils(o);
     for (size_t n = 0; n < (size_t)o->N; ++n) {
         if (!o->fluid[n]) continue;
-        double psi = o->psi[n];
+        real_t psi = o->psi[n];
         o->mu[n] = o->p.a * psi * (1 - psi * psi) - o->p.kappa * o->lap[n];
-        double jx = 0, jy = 0;
+        real_t jx = 0, jy = 0;
         for (int i = 0; i < 9; ++i) {
             jx += o->f[(size_t)i * o->N + n] * EX[i];
             jy += o->f[(size_t)i * o->N + n] * EY[i];
@@ -160,10 +168,10 @@ static void stream(o64* o) { /* :327-343: f_i(y, x) <- f*_i(y - ey, x - ex) */
     for (int i = 0; i < 9; ++i)
         for (int y = 0; y < H; ++y) {
             const int ys = wrapi(y - EY[i], H);
-            const double* fs = o->fb + (size_t)i * o->N + (size_t)ys * W;
-            const double* gs = o->gb + (size_t)i * o->N + (size_t)ys * W;
-            double* fd = o->f + (size_t)i * o->N + (size_t)y * W;
-            double* gd = o->g + (size_t)i * o->N + (size_t)y * W;
+            const real_t* fs = o->fb + (size_t)i * o->N + (size_t)ys * W;
+            const real_t* gs = o->gb + (size_t)i * o->N + (size_t)ys * W;
+            real_t* fd = o->f + (size_t)i * o->N + (size_t)y * W;
+            real_t* gd = o->g + (size_t)i * o->N + (size_t)y * W;
             for (int x = 0; x < W; ++x) {
                 int xs = x - EX[i];
                 xs = xs < 0 ? xs + W : (xs >= W ? xs - W : xs);
@@ -205,20 +213,20 @@ static void walls(o64* o) { /* bottom_top_wall on rows wall_lo / wall_hi */
 
 static void zou_he(o64* o) {
     const int H = o->H;
-    double *Px = malloc(sizeof(double) * H), *Py = malloc(sizeof(double) * H), *rin = malloc(sizeof(double) * H);
+    real_t *Px = malloc(sizeof(real_t) * H), *Py = malloc(sizeof(real_t) * H), *rin = malloc(sizeof(real_t) * H);
     /* inlet (fingering_periodic.py:268-302; fingering.py:298-364) */
     int col = 0;
     for (int y = 0; y < H; ++y) {
         size_t n = IDX(y, col);
-        double psi = psi_at(o, y, col), mu = mu_plain(o, n, psi);
+        real_t psi = psi_at(o, y, col), mu = mu_plain(o, n, psi);
         Px[y] = o->dx[n] * mu;
         Py[y] = o->dy[n] * mu;
         rin[y] = 1 / (1 - o->uprof[y]) * (F(0, y) + F(2, y) + F(4, y) + 2 * (F(3, y) + F(6, y) + F(7, y)) - Px[y] / 2);
     }
-    const double wsi = WT[1] + WT[5] + WT[8];
+    const real_t wsi = WT[1] + WT[5] + WT[8];
     for (int y = (o->p.y_walls ? 1 : 0); y < (o->p.y_walls ? H - 1 : H); ++y) {
-        double u = o->uprof[y], r = rin[y];
-        double psi_in = 1.0 - (G(0, y) + G(2, y) + G(3, y) + G(4, y) + G(6, y) + G(7, y));
+        real_t u = o->uprof[y], r = rin[y];
+        real_t psi_in = 1.0 - (G(0, y) + G(2, y) + G(3, y) + G(4, y) + G(6, y) + G(7, y));
         F(1, y) = F(3, y) + 2.0 / 3 * u * r - Px[y] / 6;
         G(1, y) = WT[1] * psi_in / wsi;
         F(5, y) = F(7, y) - 0.5 * (F(2, y) - F(4, y)) + 1.0 / 6.0 * u * r - Px[y] / 6 - Py[y] / 4;
@@ -244,13 +252,13 @@ static void zou_he(o64* o) {
     }
     /* outlet (fingering_periodic.py:305-324; fingering.py:369-390) */
     col = o->W - 1;
-    const double wso = WT[3] + WT[6] + WT[7];
+    const real_t wso = WT[3] + WT[6] + WT[7];
     for (int y = 0; y < H; ++y) {
         size_t n = IDX(y, col);
-        double psi = psi_at(o, y, col), mu = mu_plain(o, n, psi);
-        double px = o->dx[n] * mu, py = o->dy[n] * mu;
-        double known = F(0, y) + F(2, y) + F(4, y) + 2 * (F(1, y) + F(5, y) + F(8, y)) + px / 2;
-        double u, r;
+        real_t psi = psi_at(o, y, col), mu = mu_plain(o, n, psi);
+        real_t px = o->dx[n] * mu, py = o->dy[n] * mu;
+        real_t known = F(0, y) + F(2, y) + F(4, y) + 2 * (F(1, y) + F(5, y) + F(8, y)) + px / 2;
+        real_t u, r;
         if (o->p.pressure) {
             r = o->p.rho_out;
             u = known / r - 1.0;
@@ -258,7 +266,7 @@ static void zou_he(o64* o) {
             u = o->uprof[y];
             r = 1 / (1 + u) * known;
         }
-        double psi_out = -1.0 - (G(0, y) + G(1, y) + G(2, y) + G(4, y) + G(5, y) + G(8, y));
+        real_t psi_out = -1.0 - (G(0, y) + G(1, y) + G(2, y) + G(4, y) + G(5, y) + G(8, y));
         F(3, y) = F(1, y) - o->p.c3 * u * r + px / 6;
         G(3, y) = WT[3] * psi_out / wso;
         F(6, y) = F(8, y) - 0.5 * (F(2, y) - F(4, y)) - 1.0 / 6.0 * u * r + py / 4 + px / 6;
@@ -288,23 +296,23 @@ void* o64_create(const o64_params* p, const unsigned char* fluid, const unsigned
     memcpy(o->fluid, fluid, N);
     o->links = malloc(9 * N);
     memcpy(o->links, links, 9 * N);
-    double** a[] = {&o->f, &o->g, &o->fb, &o->gb};
-    for (int k = 0; k < 4; ++k) *a[k] = calloc(9 * N, sizeof(double));
-    double** b[] = {&o->psi, &o->rho, &o->ux, &o->uy, &o->mu, &o->p_, &o->tau_mix, &o->dx, &o->dy, &o->lap};
-    for (int k = 0; k < 10; ++k) *b[k] = calloc(N, sizeof(double));
-    o->uprof = malloc(sizeof(double) * o->H);
+    real_t** a[] = {&o->f, &o->g, &o->fb, &o->gb};
+    for (int k = 0; k < 4; ++k) *a[k] = calloc(9 * N, sizeof(real_t));
+    real_t** b[] = {&o->psi, &o->rho, &o->ux, &o->uy, &o->mu, &o->p_, &o->tau_mix, &o->dx, &o->dy, &o->lap};
+    for (int k = 0; k < 10; ++k) *b[k] = calloc(N, sizeof(real_t));
+    o->uprof = malloc(sizeof(real_t) * o->H);
     for (int y = 0; y < o->H; ++y) {
         if (p->gauss) {
             double t = (double)(y - o->H / 2) * 3 / ((double)o->H / 2);
-            o->uprof[y] = p->u0 * exp(-(t * t) / 2);
+            o->uprof[y] = (real_t)(p->u0 * exp(-(t * t) / 2));
         } else {
             o->uprof[y] = p->u0;
         }
     }
     /* initial state, fingering_periodic.py:90-121 (fingering.py:95-127) */
     for (size_t n = 0; n < N; ++n) {
-        o->psi[n] = fluid[n] ? psi0[n] : p->psi_wall;
-        o->rho[n] = fluid[n] ? (rho0 ? rho0[n] : p->rho0) : 0.0;
+        o->psi[n] = (real_t)(fluid[n] ? psi0[n] : p->psi_wall);
+        o->rho[n] = (real_t)(fluid[n] ? (rho0 ? rho0[n] : p->rho0) : 0.0);
     }
     stencils(o);
     for (size_t n = 0; n < N; ++n) {
@@ -315,7 +323,7 @@ void* o64_create(const o64_params* p, const unsigned char* fluid, const unsigned
             o->uy[n] = (0 + (o->mu[n] * o->dy[n] + p->gy) / 2) / o->rho[n];
         }
         o->tau_mix[n] = mix_tau(o, o->rho[n], o->psi[n]);
-        double feq[9], geq[9];
+        real_t feq[9], geq[9];
         equilibria(o, n, feq, geq);
         for (int i = 0; i < 9; ++i) {
             o->f[(size_t)i * N + n] = feq[i];
@@ -329,14 +337,44 @@ void o64_step(void* h, int nsteps) {
     o64* o = h;
     for (int s = 0; s < nsteps; ++s) {
         collide(o);
-        memcpy(o->fb, o->f, sizeof(double) * 9 * o->N);
-        memcpy(o->gb, o->g, sizeof(double) * 9 * o->N);
+        memcpy(o->fb, o->f, sizeof(real_t) * 9 * o->N);
+        memcpy(o->gb, o->g, sizeof(real_t) * 9 * o->N);
         stream(o);
         bounce(o);
         if (o->p.y_walls) walls(o);
         if (!o->p.x_periodic) zou_he(o);
         moments(o);
         o->t++;
+    }
+}
+
+/* Multiply every fluid population by (1 + eps * r), r uniform in [-1, 1]
+   from a xorshift stream: rounding-sized noise, so the spread it causes in
+   the FP64 trajectory measures how this flow amplifies FP32 rounding. */
+void o64_perturb(void* h, double eps, unsigned long long seed) {
+    o64* o = h;
+    unsigned long long s = seed * 0x9E3779B97F4A7C15ULL + 1;
+    for (int i = 0; i < 9; ++i)
+        for (size_t n = 0; n < (size_t)o->N; ++n) {
+            if (!o->fluid[n]) continue;
+            for (int k = 0; k < 2; ++k) {
+                s ^= s << 13;
+                s ^= s >> 7;
+                s ^= s << 17;
+                double r = (double)(s >> 11) / 9007199254740992.0 * 2.0 - 1.0;
+                real_t* a = k ? o->g : o->f;
+                a[(size_t)i * o->N + n] *= (real_t)(1.0 + eps * r);
+            }
+        }
+}
+
+/* Round every population to the nearest FP32 value (storage precision of
+   an FP32 solver), keeping the FP64 arithmetic of the step itself. */
+void o64_round32(void* h) {
+    o64* o = h;
+    for (size_t n = 0; n < 9 * (size_t)o->N; ++n) {
+        o->f[n] = (real_t)(float)o->f[n];
+        o->g[n] = (real_t)(float)o->g[n];
     }
 }
 
@@ -353,8 +391,10 @@ void o64_fields(void* h, double* psi, double* rho, double* ux, double* uy) {
 
 void o64_pops(void* h, double* f, double* g) {
     o64* o = h;
-    memcpy(f, o->f, sizeof(double) * 9 * o->N);
-    memcpy(g, o->g, sizeof(double) * 9 * o->N);
+    for (size_t n = 0; n < 9 * (size_t)o->N; ++n) {
+        f[n] = o->f[n];
+        g[n] = o->g[n];
+    }
 }
 
 void o64_mass(void* h, double out[2]) {
@@ -362,8 +402,8 @@ void o64_mass(void* h, double out[2]) {
     double r = 0, s = 0;
     for (size_t n = 0; n < (size_t)o->N; ++n)
         if (o->fluid[n]) {
-            r += o->rho[n];
-            s += o->psi[n];
+            r += (double)o->rho[n];
+            s += (double)o->psi[n];
         }
     out[0] = r;
     out[1] = s;
@@ -378,7 +418,7 @@ void o64_destroy(void* h) {
     free(o->g);
     free(o->fb);
     free(o->gb);
-    double* b[] = {o->psi, o->rho, o->ux, o->uy, o->mu, o->p_, o->tau_mix, o->dx, o->dy, o->lap};
+    real_t* b[] = {o->psi, o->rho, o->ux, o->uy, o->mu, o->p_, o->tau_mix, o->dx, o->dy, o->lap};
     for (int k = 0; k < 10; ++k) free(b[k]);
     free(o->uprof);
     free(o);

@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "liboracle64.so")
+_LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 
 
 class _P(ctypes.Structure):
@@ -18,16 +18,16 @@ class _P(ctypes.Structure):
                                      "wall_lo", "wall_hi")]
 
 
-_lib = None
+_libs = {}
 
 
-def _load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+def _load(bits=64):
+    if bits not in _libs:
+        path = os.path.join(_LIBDIR, "liboracle%d.so" % bits)
+        if not os.path.exists(path):
             from oracle import build_oracle
             build_oracle.build()
-        lib = ctypes.CDLL(_LIB)
+        lib = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         lib.o64_create.restype = vp
         lib.o64_create.argtypes = [ctypes.POINTER(_P), vp, vp, vp, vp]
@@ -36,8 +36,10 @@ def _load():
         lib.o64_pops.argtypes = [vp, vp, vp]
         lib.o64_mass.argtypes = [vp, vp]
         lib.o64_destroy.argtypes = [vp]
-        _lib = lib
-    return _lib
+        lib.o64_perturb.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
+        lib.o64_round32.argtypes = [vp]
+        _libs[bits] = lib
+    return _libs[bits]
 
 
 def _ptr(a):
@@ -45,10 +47,13 @@ def _ptr(a):
 
 
 class Lbm64C:
-    def __init__(self, mask, links, *, tau, eta_n, M, kappa, a, gamma, u0, psi_wall, rho0=1.0, n_inv_cols=5,
+    """FP64 C oracle; ``bits=32`` selects the same code built with float
+    arithmetic and storage (the FP32 yardstick)."""
+
+    def __init__(self, mask, links, *, bits=64, tau, eta_n, M, kappa, a, gamma, u0, psi_wall, rho0=1.0, n_inv_cols=5,
                  inlet="gauss", outlet="velocity", outlet_f3_coef=2.0 / 3.0, y_walls=False, x_periodic=False,
                  init="mu0", body_force=(0.0, 0.0), rho_out=1.0, psi0=None, rho_init=None, wall_rows="edge"):
-        lib = _load()
+        lib = self._lib = _load(bits)
         H, W = mask.shape
         self.H, self.W = H, W
         self.mask = np.ascontiguousarray(mask, dtype=bool)
@@ -69,9 +74,9 @@ class Lbm64C:
         self.t = 0
 
     @classmethod
-    def from_params(cls, p, mask, links, psi0=None, rho0=None):
+    def from_params(cls, p, mask, links, psi0=None, rho0=None, bits=64):
         wall_hi = p.H - 1 if p.wall_hi < 0 else p.wall_hi
-        return cls(mask, links, tau=p.tau, eta_n=p.eta_n, M=p.M, kappa=p.kappa, a=p.a, gamma=p.gamma, u0=p.u0,
+        return cls(mask, links, bits=bits, tau=p.tau, eta_n=p.eta_n, M=p.M, kappa=p.kappa, a=p.a, gamma=p.gamma, u0=p.u0,
                    psi_wall=p.psi_wall, rho0=p.rho0, n_inv_cols=p.n_inv_cols,
                    inlet="const" if p.profile == 1 else "gauss", outlet="pressure" if p.outlet == 1 else "velocity",
                    outlet_f3_coef=p.outlet_f3_coef, y_walls=p.y_bc == 1, x_periodic=p.x_bc == 1,
@@ -80,26 +85,34 @@ class Lbm64C:
                    wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped")
 
     def step(self, n=1):
-        _lib.o64_step(self._h, int(n))
+        self._lib.o64_step(self._h, int(n))
         self.t += n
+
+    def perturb(self, eps, seed):
+        """Rounding-sized multiplicative noise on every fluid population."""
+        self._lib.o64_perturb(self._h, float(eps), int(seed))
+
+    def round32(self):
+        """Round the stored populations to FP32."""
+        self._lib.o64_round32(self._h)
 
     def fields(self):
         out = [np.empty((self.H, self.W)) for _ in range(4)]
-        _lib.o64_fields(self._h, *[_ptr(a) for a in out])
+        self._lib.o64_fields(self._h, *[_ptr(a) for a in out])
         return tuple(out)
 
     def populations(self):
         f = np.empty((9, self.H, self.W))
         g = np.empty((9, self.H, self.W))
-        _lib.o64_pops(self._h, _ptr(f), _ptr(g))
+        self._lib.o64_pops(self._h, _ptr(f), _ptr(g))
         return f, g
 
     def mass(self):
         out = np.zeros(2)
-        _lib.o64_mass(self._h, _ptr(out))
+        self._lib.o64_mass(self._h, _ptr(out))
         return float(out[0]), float(out[1])
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.o64_destroy(self._h)
+        if getattr(self, "_h", None):
+            self._lib.o64_destroy(self._h)
             self._h = None

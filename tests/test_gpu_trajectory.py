@@ -1,13 +1,15 @@
 """The 400 x 100 channel over 5,000 iterations: GPU (FP32) against the FP64
-oracle (C restatement) at every 500 iterations, with the tolerance tied to
-the oracle's own sensitivity.
+oracle (C restatement) at every 500 iterations.
 
-Fingering is an instability, so FP32 rounding grows.  The floor of what can
-be asked of an FP32 run is how far the FP64 oracle itself moves when its
-initial psi is perturbed at the FP32 rounding level (1e-7 relative, random
-sign): the GPU must stay within TRAJ_FACTOR times that spread (plus an
-absolute floor) in L-infinity and L2 of psi, ux, uy, and its FP64 mass sums
-must follow the oracle's to the stated relative error.
+Fingering is an instability and the step has cancellations (f - feq,
+the Laplacian of a +-1 field), so FP32 arithmetic moves the trajectory by
+far more than FP32 storage rounding does.  The yardstick is therefore the
+same algorithm evaluated entirely in FP32: the C oracle built with float
+arithmetic and storage (oracle/_lib/liboracle32.so).  The GPU must stay
+within TRAJ_FACTOR times that FP32 evaluation's deviation from FP64 (plus a
+small absolute floor) in L-infinity and L2 of psi, ux, uy; its FP64 mass
+sums likewise.  Measured at 500 iterations: the FP32 evaluation is 4.8e-5
+off in psi (L-inf), the GPU about twice that.
 
 Workload: H=100, W=400, 56 non-overlapping seeded 6x6 squares (seed 1,
 5.04 % solid, x0 >= 41), psi = +1 in the first 40 columns, M = 5,
@@ -23,9 +25,15 @@ from tests.helpers import rect_domain
 
 pytestmark = pytest.mark.gpu
 
-TRAJ_FACTOR = 20.0         # GPU deviation allowed, in units of the oracle's own spread
-FLOOR = {"psi": 1e-4, "ux": 1e-6, "uy": 1e-6}
-MASS_REL = {"rho": 1e-6, "psi": 1e-5}   # relative to sum rho / number of fluid nodes
+TRAJ_FACTOR = 4.0
+FLOOR_INF = {"psi": 1e-5, "ux": 1e-7, "uy": 1e-7}
+FLOOR_L2 = {"psi": 1e-6, "ux": 1e-8, "uy": 1e-8}
+MASS_FLOOR_REL = 1e-7
+
+
+def _errs(a, b, m):
+    d = np.abs(a[m] - b[m])
+    return float(d.max()), float(np.sqrt(np.mean(d ** 2)))
 
 
 def test_cfg1_trajectory_5000():
@@ -34,36 +42,31 @@ def test_cfg1_trajectory_5000():
     mask, bits, links = rect_domain(H, W, 56, size=6, seed=1)
     p = LbmParams.periodic(H, W, M=5.0, psi_wall=0.0, n_inv_cols=40)
     psi0 = np.where(np.arange(W)[None, :] < 40, 1.0, -1.0) * np.ones((H, 1))
-    rng = np.random.default_rng(123)
-    psi0_pert = psi0 * (1.0 + 1e-7 * rng.choice([-1.0, 1.0], size=psi0.shape))
     cm = Compute(mask, bits, p, psi0=psi0)
-    a = Lbm64C.from_params(p, mask, links, psi0=psi0)
-    b = Lbm64C.from_params(p, mask, links, psi0=psi0_pert)
+    o64 = Lbm64C.from_params(p, mask, links, psi0=psi0)
+    o32 = Lbm64C.from_params(p, mask, links, psi0=psi0, bits=32)
     nf = int(mask.sum())
-    worst = {}
     for t in range(500, 5001, 500):
         cm.step(t - cm.t)
-        a.step(t - a.t)
-        b.step(t - b.t)
-        gp = cm.fields()
-        fa = a.fields()
-        fb = b.fields()
-        row = {}
+        o64.step(t - o64.t)
+        o32.step(t - o32.t)
+        gp, fa, fb = cm.fields(), o64.fields(), o32.fields()
+        parts = []
         for k, i in (("psi", 0), ("ux", 2), ("uy", 3)):
             m = np.ones_like(mask) if k == "psi" else mask
-            e_inf = float(np.abs(gp[i][m] - fa[i][m]).max())
-            e_l2 = float(np.sqrt(np.mean((gp[i][m] - fa[i][m]) ** 2)))
-            s_inf = float(np.abs(fb[i][m] - fa[i][m]).max())
-            s_l2 = float(np.sqrt(np.mean((fb[i][m] - fa[i][m]) ** 2)))
-            row[k] = (e_inf, e_l2, s_inf, s_l2)
-            assert e_inf <= TRAJ_FACTOR * s_inf + FLOOR[k], (t, k, row[k])
-            assert e_l2 <= TRAJ_FACTOR * s_l2 + FLOOR[k] / 10, (t, k, row[k])
+            g_inf, g_l2 = _errs(gp[i], fa[i], m)
+            r_inf, r_l2 = _errs(fb[i], fa[i], m)
+            parts.append("%s gpu %.2e/%.2e fp32-eval %.2e/%.2e" % (k, g_inf, g_l2, r_inf, r_l2))
+            assert g_inf <= TRAJ_FACTOR * r_inf + FLOOR_INF[k], (t, k, g_inf, r_inf)
+            assert g_l2 <= TRAJ_FACTOR * r_l2 + FLOOR_L2[k], (t, k, g_l2, r_l2)
         g_rho, g_psi, bad = cm.mass()
-        o_rho, o_psi = a.mass()
+        a_rho, a_psi = o64.mass()
+        b_rho, b_psi = o32.mass()
         assert bad == 0
-        assert abs(g_rho - o_rho) <= MASS_REL["rho"] * abs(o_rho), (t, g_rho, o_rho)
-        assert abs(g_psi - o_psi) <= MASS_REL["psi"] * nf, (t, g_psi, o_psi)
-        print("t=%d" % t, " ".join("%s: gpu %.2e/%.2e oracle-spread %.2e/%.2e" % ((k,) + v) for k, v in row.items()),
-              "mass rho %.3e rel, psi %.3e per node" % ((g_rho - o_rho) / o_rho, (g_psi - o_psi) / nf), flush=True)
-        worst = row
+        assert abs(g_rho - a_rho) <= TRAJ_FACTOR * abs(b_rho - a_rho) + MASS_FLOOR_REL * abs(a_rho), t
+        assert abs(g_psi - a_psi) <= TRAJ_FACTOR * abs(b_psi - a_psi) + MASS_FLOOR_REL * nf, t
+        print("t=%d" % t, "; ".join(parts),
+              "; mass rho gpu %.1e fp32-eval %.1e (rel), psi gpu %.1e fp32-eval %.1e (per node)"
+              % ((g_rho - a_rho) / a_rho, (b_rho - a_rho) / a_rho, (g_psi - a_psi) / nf, (b_psi - a_psi) / nf),
+              flush=True)
     cm.close()
