@@ -341,59 +341,39 @@ __device__ __forceinline__ void slot_bounce(const KParams& k, rsrc_t dst, int y,
     }
 }
 
-// One row of the march: loads for g(y+1), f(y) and flags(y+2) are issued
-// first; then psi(y+1) (after bounce-back, walls and Zou-He on g), the LDS
-// exchange of the psi row, the stencil of row y, Zou-He on f, the moments and
-// the collision (MODE 0) or the field output (MODE 1).
-//   gc, flc  g and flags of row y (in);  gn  g of row y+1 (out)
-//   fln      flags of row y+1 (in, loaded one row ahead); fln2 row y+2 (out)
-//   pm, pc   psi of rows y-1 and y (in);  pn  psi of row y+1 (out)
+// psi of in-domain row yy (wrapped) for this lane from its pulled g, after
+// bounce-back, walls and Zou-He; ghost rows / columns give their ghost value.
+__device__ __forceinline__ float psi_of_pull(const KParams& k, rsrc_t src, const Lane& ln, bool pulled, bool ghost_row,
+                                             const Rows& rr, int yy, unsigned fl, float g[9]) {
+    if (!pulled) return ghost_row ? k.c.psi_wall : ln.ghost;
+    pull_fix(src, k, 9, rr, yy, ln.x, fl, g);
+    bc_g(k, yy, (int)ln.x, g);
+    return (fl & kSolid) ? k.c.psi_wall : g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+}
+
+// Collision (MODE 0) or field output (MODE 1) of one row y of the march,
+// given its pulled f, its g and flags, and psi of rows y-1, y, y+1 in the
+// LDS ring slots jm, jc, jp.
 template <int NT, int MODE>
-__device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
-                                          float (*ring)[NT], int t, const Lane& ln, const PullOff& po,
-                                          bool interior, int y, int r,
-                                          const float gc[9], unsigned flc, unsigned fln, unsigned& fln2, float pm,
-                                          float pc, float gn[9], float& pn, double& s_rho, double& s_psi,
-                                          double& s_bad) {
-    const int yn = y + 1;
-    const bool ghost_row = k.y_walls && yn >= k.H;   // psi_wall ghost row (fingering.py:224)
-    const int ynw = (yn >= k.H) ? yn - k.H : yn;
-    // inert solid nodes need neither their own pulls nor psi from g
-    const bool skip_n = (fln & kSolid) && k.inert;
-    const bool skip_c = (flc & kSolid) && k.inert;
-    const bool pull_n = ln.inside && !ghost_row;
-    float f[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) gn[i] = f[i] = 0.f;
-    const Rows rn = rows_of(k, ynw);
-    const Rows rc = rows_of(k, y);
-    if (pull_n && !skip_n) pull_raw(src, k, 9, rn, po, gn);
-    if (interior && !skip_c) pull_raw(src, k, 0, rc, po, f);
-    fln2 = flags_of(k, b, ln, y + 2);
-    if (pull_n) {
-        pull_fix(src, k, 9, rn, ynw, ln.x, fln, gn);
-        bc_g(k, ynw, (int)ln.x, gn);
-        pn = (fln & kSolid) ? k.c.psi_wall
-                            : gn[0] + gn[1] + gn[2] + gn[3] + gn[4] + gn[5] + gn[6] + gn[7] + gn[8];
-    } else {
-        pn = ghost_row ? k.c.psi_wall : ln.ghost;
-    }
-    ring[(r + 2) & 3][t] = pn;
-    __syncthreads();
-    if (!interior) return;
-    const float* Rm = ring[r & 3];
-    const float* Rc = ring[(r + 1) & 3];
-    const float* Rp = ring[(r + 2) & 3];
-    const float n[9] = {pc, Rc[t + 1], pn, Rc[t - 1], pm, Rp[t + 1], Rp[t - 1], Rm[t - 1], Rm[t + 1]};
+__device__ __forceinline__ void finish_row(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
+                                           float (*ring)[NT], int t, const Lane& ln, const PullOff& po, int y,
+                                           int jm, int jc, int jp, float f[9], const float gc[9], unsigned flc,
+                                           double& s_rho, double& s_psi, double& s_bad) {
+    const float* Rm = ring[jm];
+    const float* Rc = ring[jc];
+    const float* Rp = ring[jp];
+    const float pc = Rc[t];
+    const float n[9] = {pc, Rc[t + 1], Rp[t], Rc[t - 1], Rm[t], Rp[t + 1], Rp[t - 1], Rm[t - 1], Rm[t + 1]};
     float dpx, dpy, lap;
     fdn::stencil(n, dpx, dpy, lap);
+    const Rows rc = rows_of(k, y);
     pull_fix(src, k, 0, rc, y, ln.x, flc, f);
     bc_f(k, b, src, y, (int)ln.x, f);
     float g[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) g[i] = gc[i];
-    const unsigned o = rc.c + ln.x;
     const bool solid = (flc & kSolid) != 0;
+    const bool skip = solid && k.inert;
     if (MODE == 0) {
         if (!k.x_periodic && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
             const float mu = fdn::chem_pot(pc, lap, k.c);
@@ -405,7 +385,7 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
             const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
             fdn::collide(f, g, m, k.c);
         }
-        if (!skip_c) {
+        if (!skip) {
             // node (y, x) of plane i: the pull offset of direction i moved back by e_ix
             const int rowf = rc.c * 4, rowg = (rc.c + 9 * k.Q) * 4;
 #pragma unroll
@@ -417,6 +397,7 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
         }
         if (k.inert) slot_bounce(k, dst, y, ln.x, flc, f, g);
     } else {
+        const unsigned o = rc.c + ln.x;
         float rho = 0.f, ux = 0.f, uy = 0.f;
         if (!solid) {
             const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
@@ -435,11 +416,65 @@ __device__ __forceinline__ void march_row(const KParams& k, const KBufs& b, rsrc
             // inert solid nodes carry no populations; report zeros there
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
-                b.dst[i * k.Q + o] = skip_c ? 0.f : f[i];
-                b.dst[(9 + i) * k.Q + o] = skip_c ? 0.f : g[i];
+                b.dst[i * k.Q + o] = skip ? 0.f : f[i];
+                b.dst[(9 + i) * k.Q + o] = skip ? 0.f : g[i];
             }
         }
     }
+}
+
+// Two rows of the march (TWO) or the last single row.  All streaming loads
+// of the pair - g of rows y+1 and y+2 for psi, f of rows y and y+1 - and the
+// flags two rows ahead are issued before anything waits; then psi of rows
+// y+1, y+2 goes into the LDS ring (8 slots, one barrier per pair), and rows
+// y and y+1 are collided.
+//   g0   g of row y (in) -> g of row y+2 (out, TWO) or y+1 (out, !TWO)
+//   fl   flags of rows y .. y+4 in fl[0..4]; fl[3], fl[4] are loaded here
+//   j    ring slot index of row y (row y0-1 has index 0)
+template <int NT, int MODE, bool TWO>
+__device__ __forceinline__ void march_pair(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
+                                           float (*ring)[NT], int t, const Lane& ln, const PullOff& po,
+                                           bool interior, int y, int j, float g0[9], unsigned fl[5], double& s_rho,
+                                           double& s_psi, double& s_bad) {
+    auto wrapy = [&](int v) { return v >= k.H ? v - k.H : v; };
+    const int y1 = y + 1, y2 = y + 2;
+    const bool gr1 = k.y_walls && y1 >= k.H;   // psi_wall ghost rows (fingering.py:224)
+    const bool gr2 = k.y_walls && y2 >= k.H;
+    const bool in1 = ln.inside && !gr1 && !((fl[1] & kSolid) && k.inert);
+    const bool in2 = TWO && ln.inside && !gr2 && !((fl[2] & kSolid) && k.inert);
+    const bool fa_on = interior && !((fl[0] & kSolid) && k.inert);
+    const bool fb_on = TWO && interior && !((fl[1] & kSolid) && k.inert);
+    const Rows r0 = rows_of(k, y), r1 = rows_of(k, wrapy(y1)), r2 = rows_of(k, wrapy(y2));
+    float g1[9], g2[9], fa[9], fb[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g1[i] = g2[i] = fa[i] = fb[i] = 0.f;
+    if (in1) pull_raw(src, k, 9, r1, po, g1);
+    if (in2) pull_raw(src, k, 9, r2, po, g2);
+    if (fa_on) pull_raw(src, k, 0, r0, po, fa);
+    if (fb_on) pull_raw(src, k, 0, r1, po, fb);
+    if (TWO) {
+        fl[3] = flags_of(k, b, ln, y + 3);
+        fl[4] = flags_of(k, b, ln, y + 4);
+    } else {
+        fl[3] = flags_of(k, b, ln, y + 3);
+    }
+    const bool p1 = ln.inside && !gr1, p2 = TWO && ln.inside && !gr2;
+    const float psi1 = psi_of_pull(k, src, ln, p1 && in1, gr1, r1, wrapy(y1), fl[1], g1) ;
+    ring[(j + 2) & 7][t] = (p1 && !in1 && !gr1) ? k.c.psi_wall : psi1;
+    if (TWO) {
+        const float psi2 = psi_of_pull(k, src, ln, p2 && in2, gr2, r2, wrapy(y2), fl[2], g2);
+        ring[(j + 3) & 7][t] = (p2 && !in2 && !gr2) ? k.c.psi_wall : psi2;
+    }
+    __syncthreads();
+    if (interior) {
+        finish_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, y, j & 7, (j + 1) & 7, (j + 2) & 7, fa, g0, fl[0],
+                             s_rho, s_psi, s_bad);
+        if (TWO)
+            finish_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, y1, (j + 1) & 7, (j + 2) & 7, (j + 3) & 7, fb, g1,
+                                 fl[1], s_rho, s_psi, s_bad);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g0[i] = TWO ? g2[i] : g1[i];
 }
 
 // psi and g of row y for this lane (the march prologue); fl = flags_of(row y).
@@ -469,7 +504,7 @@ template <int WX, int MODE>
 __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     constexpr int NT = 64 * WX;
     constexpr int SW = NT - 2;
-    __shared__ float ring[4][NT];
+    __shared__ float ring[8][NT];
     const int t = threadIdx.x;
     const int strip = blockIdx.x % k.nstrips;
     const int chunk = blockIdx.x / k.nstrips;
@@ -482,35 +517,27 @@ __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     const unsigned bytes = 18u * (unsigned)k.Q * 4u;
     const rsrc_t src = make_rsrc(b.src, bytes);
     const rsrc_t dst = make_rsrc(b.dst, b.dst ? bytes : 0u);
-    float gA[9], gB[9];
     double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
-    // rolling flags: fA row y, fB row y+1, fC row y+2 (loaded one row ahead)
-    unsigned fA = flags_of(k, b, ln, y0);
-    unsigned fB = flags_of(k, b, ln, y0 + 1);
-    unsigned fC;
     const PullOff po = pull_offsets(k, ln.x);
-    float pm = psi_row(k, src, ln, po, y0 - 1, flags_of(k, b, ln, y0 - 1), gB);
-    float pc = psi_row(k, src, ln, po, y0, fA, gA);
-    float pn;
-    ring[0][t] = pm;
-    ring[1][t] = pc;
-    // two rows per trip so the rolling g / psi / flag rows swap roles statically
-    for (int y = y0; y < y1; y += 2) {
-        march_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, gA, fA, fB, fC, pm, pc, gB, pn,
-                            s_rho, s_psi, s_bad);
-        pm = pc;
-        pc = pn;
-        if (y + 1 < y1) {
-            march_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, interior, y + 1, y + 1 - y0, gB, fB, fC, fA, pm,
-                                pc, gA, pn, s_rho, s_psi, s_bad);
-            pm = pc;
-            pc = pn;
-            // the second row wrote flags of row y+3 into fA: rows y+2, y+3 -> fA, fB
-            const unsigned f3 = fA;
-            fA = fC;
-            fB = f3;
-        }
+    // rolling flags of rows y .. y+4
+    unsigned fl[5];
+    fl[0] = flags_of(k, b, ln, y0);
+    fl[1] = flags_of(k, b, ln, y0 + 1);
+    fl[2] = flags_of(k, b, ln, y0 + 2);
+    float g0[9], gm[9];
+    ring[0][t] = psi_row(k, src, ln, po, y0 - 1, flags_of(k, b, ln, y0 - 1), gm);
+    ring[1][t] = psi_row(k, src, ln, po, y0, fl[0], g0);
+    int y = y0;
+    for (; y + 1 < y1; y += 2) {
+        march_pair<NT, MODE, true>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, g0, fl, s_rho, s_psi,
+                                   s_bad);
+        fl[0] = fl[2];
+        fl[1] = fl[3];
+        fl[2] = fl[4];
     }
+    if (y < y1)
+        march_pair<NT, MODE, false>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, g0, fl, s_rho, s_psi,
+                                    s_bad);
     if (MODE == 1) block_sum3<NT>(s_rho, s_psi, s_bad, b.partials + 3 * blockIdx.x);
 }
 
@@ -604,11 +631,18 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
     if (PRE) block_sum3<256>(s_rho, s_psi, s_bad, b.partials + 3 * (blockIdx.y * gridDim.x + blockIdx.x));
 }
 
-// Plain streaming copy (16 B per lane per trip), the HBM yardstick.
+// Plain streaming copy, the HBM yardstick: each lane moves four
+// independent 16-byte vectors per trip (n must be a multiple of 4 * grid).
 __global__ __launch_bounds__(256) void fd_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b,
                                                       int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 3 * stride < n; i += 4 * stride) {
+        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
 }
 
 __global__ __launch_bounds__(256) void reduce_partials(const double* __restrict__ part, int n,
@@ -1034,8 +1068,10 @@ int64_t fd_steps_done(const fd_handle* h) { return h ? h->N : -1; }
 int fd_solid_inert(const fd_handle* h) { return h ? (h->inert ? 1 : 0) : -1; }
 
 int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy) {
-    if (nbytes < 16 || reps < 1 || !ms_per_copy) return fail(FD_E_ARG, "bad copy probe arguments");
-    const int64_t n = nbytes / 16;
+    // whole trips of 8192 x 256 lanes x 4 vectors x 16 B (32 MiB)
+    const int64_t trip = (int64_t)8192 * 256 * 4;
+    if (nbytes < trip * 16 || reps < 1 || !ms_per_copy) return fail(FD_E_ARG, "copy probe needs >= 32 MiB");
+    const int64_t n = nbytes / 16 / trip * trip;
     float4 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
