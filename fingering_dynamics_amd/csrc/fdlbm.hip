@@ -35,16 +35,17 @@
 namespace {
 
 // Per-node flags (uint32):
-//   bits 0-7   slot links: direction i is bounced back through the solid
-//              upstream node's own slot (written by this node's collision)
-//   bits 8-15  explicit links: direction i reloaded from this node's own
-//              post-collision opposite population when pulled
+//   bits 0-7   links: direction i (bit i-1) is bounced back, i.e. pulled
+//              from the node's own post-collision opposite population
+//              (half-way bounce-back, bounce_back.py:25-167); the top and
+//              bottom walls (fingering.py:432-451, 573) are links of their
+//              rows, set by the host
 //   bit 16     solid
 // In the inert-solid mode (leak-free geometry) solid nodes are never read
-// or written; otherwise every link is explicit and solid nodes stream their
-// populations on unchanged (store-and-forward, fingering_periodic.py:455-466).
-constexpr unsigned kSlotLinks = 0xFFu;
-constexpr unsigned kExplShift = 8;
+// and store zeros; otherwise they stream their populations on unchanged
+// (store-and-forward, fingering_periodic.py:455-466).  Either way every node
+// of a row is stored, so the 18 planes are written in whole cache lines.
+constexpr unsigned kLinks = 0xFFu;
 constexpr unsigned kSolid = 1u << 16;
 // A vector offset beyond any descriptor's range: such buffer loads return 0
 // and such stores are dropped, without a memory access.
@@ -53,9 +54,9 @@ constexpr unsigned kOOB = 0x7FFFFFFCu;
 struct KParams {
     int H, W, Wp;
     int Q;          // plane stride in floats (H * Wp)
-    int rows;       // rows per workgroup chunk
-    int nstrips;    // workgroup strips across x
-    int y_walls, x_periodic, pressure, wall_lo, wall_hi;
+    int rows;       // rows per wave
+    int nsb;        // workgroups across x (each covers WPB strips of 64 columns)
+    int y_walls, x_periodic, pressure;
     int inert;      // solid nodes are never read (leak-free geometry)
     float rho_out, c3;
     fdn::Consts<float> c;
@@ -69,13 +70,13 @@ struct KBufs {
     float* __restrict__ side_out;
     const float* __restrict__ uprof;   // [H] inlet/outlet velocity
     float* __restrict__ fields;        // [4][H][Wp] psi, rho, ux, uy
-    double* __restrict__ partials;     // [gridDim][3]
+    double* __restrict__ partials;     // per wave [3]
 };
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 // Row geometry shared by every lane of a wave (y is uniform across a
-// workgroup): the three source rows of a pull, with periodic wrap.
+// wave): the three source rows of a pull, with periodic wrap.
 struct Rows {
     int s, c, n;  // y-1, y, y+1 (wrapped), already multiplied by the pitch
 };
@@ -89,110 +90,61 @@ __device__ __forceinline__ Rows rows_of(const KParams& k, int y) {
 }
 
 // Buffer-resource access to a population set: the descriptor covers the
-// whole 18-plane allocation; the vector offset is the lane's column in bytes
-// and the scalar offset the (plane, row, column shift) part, which is the
-// same for every lane of the workgroup.  Out-of-range offsets read 0.
+// whole 18-plane allocation; the vector offset carries the lane's column
+// (and, for pulls, the plane) in bytes, the scalar offset the part that is
+// the same for every lane of the wave.  Out-of-range offsets read 0.
 using rsrc_t = __amdgpu_buffer_rsrc_t;
 constexpr int kRsrcWord3 = 0x00020000;
 
 __device__ __forceinline__ rsrc_t make_rsrc(const float* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kRsrcWord3);
 }
-// The b32 builtins move raw bits (unsigned); the float is bit-cast, never converted.
-__device__ __forceinline__ float bld(rsrc_t r, unsigned vbyte, int sidx) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vbyte, sidx * 4, 0));
-}
-__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vbyte, int sidx) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vbyte, sidx * 4, 0);
-}
 
 // Per-lane byte offsets of the nine pull sources inside the f planes:
-// direction i reads plane i at column x - e_ix, i.e. ((x - e_ix) + i*Q)*4.
-// They do not change along the march, so they live in vector registers and
-// each load only adds a row offset in a scalar register (g planes: + 9*Q*4).
-// x - 1 and x + 1 are not wrapped: at x = 0 and x = W-1 the affected
-// directions (1, 5, 8 and 3, 6, 7) are either rewritten by Zou-He or, for
-// periodic x, reloaded by pull_fix; the -1 shifts only occur in planes >= 1,
-// so every offset is non-negative and inside the allocation.
+// direction i reads plane i at column x - e_ix, i.e. ((x - e_ix) + i*Q)*4,
+// wrapped in x when x is periodic (np.roll semantics).  For Zou-He x the
+// column -1 / W reads of the first / last column land inside the
+// allocation (planes >= 1, padding columns) and are rewritten by Zou-He.
 struct PullOff {
     unsigned v[9];
+    unsigned xb;  // the lane's own column in bytes
 };
 
 __device__ __forceinline__ PullOff pull_offsets(const KParams& k, unsigned x) {
     PullOff po;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) po.v[i] = (unsigned)((int)x - fdn::EX[i] + i * k.Q) * 4u;
+    for (int i = 0; i < 9; ++i) {
+        int xs = (int)x - fdn::EX[i];
+        if (k.x_periodic) xs = wrap(xs, k.W);
+        po.v[i] = (unsigned)(xs + i * k.Q) * 4u;
+    }
+    po.xb = x * 4u;
     return po;
 }
 
 // Streaming loads of one distribution (plane base P0 = 0 for f, 9 for g)
-// into the node whose pull offsets are po, at the rows of r.
-__device__ __forceinline__ void pull_raw(rsrc_t src, const KParams& k, int P0, const Rows& r, const PullOff& po,
-                                         float v[9]) {
+// into the node whose pull offsets are po, at the rows of r, with the
+// node's bounced-back links already applied: a linked direction i reads
+// the node's own post-collision f_opp(i) (plane opp(i), row y) instead of
+// its upstream neighbour - the same single load at another address.  Lanes
+// with !on read 0 without a memory access: loads are predicated by offset,
+// never by a branch, so every path through the march issues the same loads
+// and the waits for them stay exact.
+__device__ __forceinline__ void pull(rsrc_t src, const KParams& k, int P0, const Rows& r, const PullOff& po,
+                                     unsigned links, bool on, float v[9]) {
     const int gq = P0 * k.Q;
-    const int rs = r.s + gq, rc = r.c + gq, rn = r.n + gq;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const int row = fdn::EY[i] > 0 ? rs : (fdn::EY[i] < 0 ? rn : rc);
-        v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(src, po.v[i], row * 4, 0));
+        const int row = fdn::EY[i] > 0 ? r.s : (fdn::EY[i] < 0 ? r.n : r.c);
+        unsigned vo = po.v[i];
+        if (i > 0) {
+            // own node, plane opp(i), row y, relative to this load's scalar offset
+            const unsigned d = (unsigned)(fdn::OPP[i] * k.Q + r.c - row) * 4u;
+            vo = ((links >> (i - 1)) & 1u) ? po.xb + d : vo;
+        }
+        v[i] = __builtin_bit_cast(float,
+                                  __builtin_amdgcn_raw_buffer_load_b32(src, on ? vo : kOOB, (row + gq) * 4, 0));
     }
-}
-
-// Periodic-x wrap for the first and last column, then the node's
-// bounce-back links and the walls; all overwrite values with the node's own
-// post-collision populations (bounce_back.py:25-167, fingering.py:432-451).
-__device__ __forceinline__ void pull_fix(rsrc_t src, const KParams& k, int P0, const Rows& r, int y,
-                                         unsigned x, unsigned fl, float v[9]) {
-    const int b = P0 * k.Q;
-    const int Q = k.Q;
-    const unsigned xb = x * 4u;
-    if (k.x_periodic) {
-        if (x == 0) {
-            const unsigned wb = (unsigned)(k.W - 1) * 4u;
-            v[1] = bld(src, wb, b + Q + r.c);
-            v[5] = bld(src, wb, b + 5 * Q + r.s);
-            v[8] = bld(src, wb, b + 8 * Q + r.n);
-        }
-        if (x == (unsigned)k.W - 1) {
-            v[3] = bld(src, 0u, b + 3 * Q + r.c);
-            v[6] = bld(src, 0u, b + 6 * Q + r.s);
-            v[7] = bld(src, 0u, b + 7 * Q + r.n);
-        }
-    }
-    // explicit bounce-back: all eight directions are loaded into temporaries
-    // (out-of-range offset for lanes without that link, so no memory access)
-    // and selected afterwards, so no load waits on an earlier one; waves
-    // without any explicit link skip the block.
-    const unsigned links = (fl >> kExplShift) & 0xFFu;
-    if (__any(links != 0u)) {
-        float t[9];
-#pragma unroll
-        for (int i = 1; i < 9; ++i) {
-            const bool on = (links >> (i - 1)) & 1u;
-            t[i] = bld(src, on ? xb : kOOB, b + fdn::OPP[i] * Q + r.c);
-        }
-#pragma unroll
-        for (int i = 1; i < 9; ++i) v[i] = ((links >> (i - 1)) & 1u) ? t[i] : v[i];
-    }
-    if (k.y_walls) {
-        if (y == k.wall_lo) {
-            v[2] = bld(src, xb, b + 4 * Q + r.c);
-            v[5] = bld(src, xb, b + 7 * Q + r.c);
-            v[6] = bld(src, xb, b + 8 * Q + r.c);
-        }
-        if (y == k.wall_hi) {
-            v[4] = bld(src, xb, b + 2 * Q + r.c);
-            v[7] = bld(src, xb, b + 5 * Q + r.c);
-            v[8] = bld(src, xb, b + 6 * Q + r.c);
-        }
-    }
-}
-
-__device__ __forceinline__ void pull(rsrc_t src, const KParams& k, int P0, int y, unsigned x, unsigned fl,
-                                     float v[9]) {
-    const Rows r = rows_of(k, y);
-    pull_raw(src, k, P0, r, pull_offsets(k, x), v);
-    pull_fix(src, k, P0, r, y, x, fl, v);
 }
 
 // Zou-He (and walled corner) update of g at an inlet/outlet node.  The
@@ -226,7 +178,7 @@ __device__ __forceinline__ void bc_g(const KParams& k, int y, int x, float g[9])
 // Inlet density of row r (walled corner nodes use the adjacent row's).
 __device__ __forceinline__ float inlet_rho_row(const KParams& k, const KBufs& b, rsrc_t src, int r) {
     float f[9];
-    pull(src, k, 0, r, 0u, b.flags[r * k.Wp], f);  // column 0 is never solid in the inert mode
+    pull(src, k, 0, rows_of(k, r), pull_offsets(k, 0u), b.flags[r * k.Wp] & kLinks, true, f);
     return fdn::inlet_rho(f, b.uprof[r], b.side_in[r]);
 }
 
@@ -285,6 +237,20 @@ __device__ __forceinline__ void block_sum3(double s0, double s1, double s2, doub
     }
 }
 
+__device__ __forceinline__ void wave_sum3(double s0, double s1, double s2, double* out, int lane) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    if (lane == 0) {
+        out[0] = s0;
+        out[1] = s1;
+        out[2] = s2;
+    }
+}
+
 // Where a lane's column sits: inside the domain, on a ghost column of the
 // psi stencil, or unused (beyond x = W in the last strip).
 struct Lane {
@@ -311,93 +277,71 @@ __device__ __forceinline__ Lane lane_of(const KParams& k, int x) {
 }
 
 // Flags of row y (wrapped) for this lane; ghost rows and ghost columns read
-// as solid so no pull is issued for them.
+// as solid.  The load is unconditional (at a clamped, valid address) and
+// the ghost value selected afterwards.
 __device__ __forceinline__ unsigned flags_of(const KParams& k, const KBufs& b, const Lane& ln, int y) {
-    if (!ln.inside || (k.y_walls && (y < 0 || y >= k.H))) return kSolid;
-    y = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
-    return b.flags[y * k.Wp + ln.x];
+    const bool ghost = !ln.inside || (k.y_walls && (y < 0 || y >= k.H));
+    const int yw = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
+    const unsigned v = b.flags[yw * k.Wp + (ln.inside ? ln.x : 0u)];
+    return ghost ? kSolid : v;
 }
 
-// Bounce-back through the solid neighbour's slot (inert mode): direction i
-// of node x is bounced when x - e_i is solid, so the next pull of direction
-// i at x reads plane i at x - e_i, a slot nothing else reads or writes.
-// Storing the post-collision opposite population there makes that pull
-// return f*_opp(i)(x), which is what half-way bounce-back prescribes.
-__device__ __forceinline__ void slot_bounce(const KParams& k, rsrc_t dst, int y, unsigned x, unsigned fl,
-                                            const float f[9], const float g[9]) {
-    const unsigned links = fl & kSlotLinks;
-    if (!__any(links != 0u)) return;
-#pragma unroll
-    for (int i = 1; i < 9; ++i) {
-        const bool on = (links >> (i - 1)) & 1u;
-        if (!__any(on)) continue;
-        int ys = y - fdn::EY[i];
-        ys = (ys < 0) ? ys + k.H : (ys >= k.H ? ys - k.H : ys);
-        int xs = (int)x - fdn::EX[i];
-        xs = (xs < 0) ? xs + k.W : (xs >= k.W ? xs - k.W : xs);
-        const unsigned vb = on ? (unsigned)xs * 4u : kOOB;
-        bst(f[fdn::OPP[i]], dst, vb, i * k.Q + ys * k.Wp);
-        bst(g[fdn::OPP[i]], dst, vb, (9 + i) * k.Q + ys * k.Wp);
-    }
-}
-
-// psi of in-domain row yy (wrapped) for this lane from its pulled g, after
-// bounce-back, walls and Zou-He; ghost rows / columns give their ghost value.
-__device__ __forceinline__ float psi_of_pull(const KParams& k, rsrc_t src, const Lane& ln, bool pulled, bool ghost_row,
-                                             const Rows& rr, int yy, unsigned fl, float g[9]) {
-    if (!pulled) return ghost_row ? k.c.psi_wall : ln.ghost;
-    pull_fix(src, k, 9, rr, yy, ln.x, fl, g);
-    bc_g(k, yy, (int)ln.x, g);
-    return (fl & kSolid) ? k.c.psi_wall : g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
-}
-
-// Collision (MODE 0) or field output (MODE 1) of one row y of the march,
-// given its pulled f, its g and flags, and psi of rows y-1, y, y+1 in the
-// LDS ring slots jm, jc, jp.
-template <int NT, int MODE>
-__device__ __forceinline__ void finish_row(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
-                                           float (*ring)[NT], int t, const Lane& ln, const PullOff& po, int y,
-                                           int jm, int jc, int jp, float f[9], const float gc[9], unsigned flc,
-                                           double& s_rho, double& s_psi, double& s_bad) {
-    const float* Rm = ring[jm];
-    const float* Rc = ring[jc];
-    const float* Rp = ring[jp];
-    const float pc = Rc[t];
-    const float n[9] = {pc, Rc[t + 1], Rp[t], Rc[t - 1], Rm[t], Rp[t + 1], Rp[t - 1], Rm[t - 1], Rm[t + 1]};
+// Collision (MODE 0) or field output (MODE 1) of node (y, ln.x), given its
+// pulled f (links applied, Zou-He not yet), its finished g and flags, and
+// psi of the node and its eight neighbours in n[] (C, E, N, W, S, NE, NW,
+// SW, SE).
+//   on   the node is the lane's own (false beyond the domain: nothing is
+//        written for it)
+template <int MODE>
+__device__ __forceinline__ void finish_node(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
+                                            const Lane& ln, const PullOff& po, int y, bool on, const float n[9],
+                                            float f[9], const float gc[9], unsigned flc, double& s_rho,
+                                            double& s_psi, double& s_bad) {
+    const float pc = n[0];
     float dpx, dpy, lap;
     fdn::stencil(n, dpx, dpy, lap);
-    const Rows rc = rows_of(k, y);
-    pull_fix(src, k, 0, rc, y, ln.x, flc, f);
+    const int rc = y * k.Wp;
     bc_f(k, b, src, y, (int)ln.x, f);
     float g[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) g[i] = gc[i];
     const bool solid = (flc & kSolid) != 0;
-    const bool skip = solid && k.inert;
     if (MODE == 0) {
-        if (!k.x_periodic && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
+        if (on && !k.x_periodic && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
             const float mu = fdn::chem_pot(pc, lap, k.c);
             const int s = (ln.x == 0) ? 0 : 2;
             b.side_out[s * k.H + y] = dpx * mu;
             b.side_out[(s + 1) * k.H + y] = dpy * mu;
         }
-        if (!solid) {
+        // collided on every lane (values of solid lanes are discarded by the
+        // select), so the pulled registers are read on every path and the
+        // stores below never wait for an older store to complete
+        {
             const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
-            fdn::collide(f, g, m, k.c);
-        }
-        if (!skip) {
-            // node (y, x) of plane i: the pull offset of direction i moved back by e_ix
-            const int rowf = rc.c * 4, rowg = (rc.c + 9 * k.Q) * 4;
+            float fo[9], go[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
-                const unsigned vo = po.v[i] + (unsigned)(fdn::EX[i] * 4);
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, rowf, 0);
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo, rowg, 0);
+                fo[i] = f[i];
+                go[i] = g[i];
+            }
+            fdn::collide(fo, go, m, k.c);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                f[i] = solid ? f[i] : fo[i];
+                g[i] = solid ? g[i] : go[i];
             }
         }
-        if (k.inert) slot_bounce(k, dst, y, ln.x, flc, f, g);
-    } else {
-        const unsigned o = rc.c + ln.x;
+        // every node of the row is stored (inert solid nodes store their
+        // zeros), predicated by offset like the loads
+        const unsigned vo = on ? po.xb : kOOB;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, (rc + i * k.Q) * 4, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo,
+                                                  (rc + (9 + i) * k.Q) * 4, 0);
+        }
+    } else if (on) {
+        const unsigned o = rc + ln.x;
         float rho = 0.f, ux = 0.f, uy = 0.f;
         if (!solid) {
             const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
@@ -413,104 +357,100 @@ __device__ __forceinline__ void finish_row(const KParams& k, const KBufs& b, rsr
         b.fields[2 * k.Q + o] = ux;
         b.fields[3 * k.Q + o] = uy;
         if (b.dst) {
-            // inert solid nodes carry no populations; report zeros there
+            // inert solid nodes carry no populations (they were not pulled: zeros)
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
-                b.dst[i * k.Q + o] = skip ? 0.f : f[i];
-                b.dst[(9 + i) * k.Q + o] = skip ? 0.f : g[i];
+                b.dst[i * k.Q + o] = f[i];
+                b.dst[(9 + i) * k.Q + o] = g[i];
             }
         }
     }
 }
 
-// Two rows of the march (TWO) or the last single row.  All streaming loads
-// of the pair - g of rows y+1 and y+2 for psi, f of rows y and y+1 - and the
-// flags two rows ahead are issued before anything waits; then psi of rows
-// y+1, y+2 goes into the LDS ring (8 slots, one barrier per pair), and rows
-// y and y+1 are collided.
-//   g0   g of row y (in) -> g of row y+2 (out, TWO) or y+1 (out, !TWO)
-//   fl   flags of rows y .. y+4 in fl[0..4]; fl[3], fl[4] are loaded here
-//   j    ring slot index of row y (row y0-1 has index 0)
-template <int NT, int MODE, bool TWO>
-__device__ __forceinline__ void march_pair(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
-                                           float (*ring)[NT], int t, const Lane& ln, const PullOff& po,
-                                           bool interior, int y, int j, float g0[9], unsigned fl[5], double& s_rho,
-                                           double& s_psi, double& s_bad) {
-    auto wrapy = [&](int v) { return v >= k.H ? v - k.H : v; };
-    const int y1 = y + 1, y2 = y + 2;
-    const bool gr1 = k.y_walls && y1 >= k.H;   // psi_wall ghost rows (fingering.py:224)
-    const bool gr2 = k.y_walls && y2 >= k.H;
-    const bool in1 = ln.inside && !gr1 && !((fl[1] & kSolid) && k.inert);
-    const bool in2 = TWO && ln.inside && !gr2 && !((fl[2] & kSolid) && k.inert);
-    const bool fa_on = interior && !((fl[0] & kSolid) && k.inert);
-    const bool fb_on = TWO && interior && !((fl[1] & kSolid) && k.inert);
-    const Rows r0 = rows_of(k, y), r1 = rows_of(k, wrapy(y1)), r2 = rows_of(k, wrapy(y2));
-    float g1[9], g2[9], fa[9], fb[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g1[i] = g2[i] = fa[i] = fb[i] = 0.f;
-    if (in1) pull_raw(src, k, 9, r1, po, g1);
-    if (in2) pull_raw(src, k, 9, r2, po, g2);
-    if (fa_on) pull_raw(src, k, 0, r0, po, fa);
-    if (fb_on) pull_raw(src, k, 0, r1, po, fb);
-    if (TWO) {
-        fl[3] = flags_of(k, b, ln, y + 3);
-        fl[4] = flags_of(k, b, ln, y + 4);
-    } else {
-        fl[3] = flags_of(k, b, ln, y + 3);
-    }
-    const bool p1 = ln.inside && !gr1, p2 = TWO && ln.inside && !gr2;
-    const float psi1 = psi_of_pull(k, src, ln, p1 && in1, gr1, r1, wrapy(y1), fl[1], g1) ;
-    ring[(j + 2) & 7][t] = (p1 && !in1 && !gr1) ? k.c.psi_wall : psi1;
-    if (TWO) {
-        const float psi2 = psi_of_pull(k, src, ln, p2 && in2, gr2, r2, wrapy(y2), fl[2], g2);
-        ring[(j + 3) & 7][t] = (p2 && !in2 && !gr2) ? k.c.psi_wall : psi2;
-    }
-    __syncthreads();
-    if (interior) {
-        finish_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, y, j & 7, (j + 1) & 7, (j + 2) & 7, fa, g0, fl[0],
-                             s_rho, s_psi, s_bad);
-        if (TWO)
-            finish_row<NT, MODE>(k, b, src, dst, ring, t, ln, po, y1, (j + 1) & 7, (j + 2) & 7, (j + 3) & 7, fb, g1,
-                                 fl[1], s_rho, s_psi, s_bad);
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g0[i] = TWO ? g2[i] : g1[i];
+// ---------------------------------------------------------------- wave march
+// One wavefront per aligned 64-column strip and chunk of rows; no LDS and no
+// barrier, so every wave streams on its own.  Lane l of strip s owns column
+// x = 64 s + l.  psi of the two columns beside the strip (x0 - 1, x0 + 64) is
+// pulled by lanes 0 and 63 in addition to their own column; psi of a lane's
+// left and right neighbours comes by cross-lane shuffle.  The march keeps
+// psi of rows y - 1, y, y + 1 and g of row y in registers.
+//
+// Each row issues all of its loads first - the flags two rows ahead, g of
+// row y + 1, f of row y, the halo g - and every loaded value is consumed
+// within the row, so no load is outstanding when the row's stores go out
+// and the next row's loads follow them without a wait.  The flags are
+// fetched a row ahead because the link bits choose the load addresses.
+
+// psi of row yy at the lane's column and at its left / right neighbours
+struct PsiRow {
+    float c, l, r;
+};
+
+// Whether g of node (yy, ln) is loaded: an in-domain row and column, and
+// not an inert solid (flags fl of that node).
+__device__ __forceinline__ bool g_on(const KParams& k, const Lane& ln, int yy, unsigned fl) {
+    return ln.inside && !(k.y_walls && (yy < 0 || yy >= k.H)) && !((fl & kSolid) && k.inert);
 }
 
-// psi and g of row y for this lane (the march prologue); fl = flags_of(row y).
-__device__ __forceinline__ float psi_row(const KParams& k, rsrc_t src, const Lane& ln, const PullOff& po, int y,
-                                         unsigned fl, float g[9]) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g[i] = 0.f;
-    if (k.y_walls && (y < 0 || y >= k.H)) return k.c.psi_wall;
-    if (!ln.inside) return ln.ghost;
-    if ((fl & kSolid) && k.inert) return k.c.psi_wall;
-    y = (y < 0) ? y + k.H : (y >= k.H ? y - k.H : y);
-    const Rows r = rows_of(k, y);
-    pull_raw(src, k, 9, r, po, g);
-    pull_fix(src, k, 9, r, y, ln.x, fl, g);
-    bc_g(k, y, (int)ln.x, g);
-    if (fl & kSolid) return k.c.psi_wall;
-    return g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+// Zou-He on the g pulled for row yy (lanes with on), then psi of the node:
+// psi_wall on ghost rows and solid nodes, the ghost value on ghost columns,
+// else the sum of g (fingering_periodic.py:470).
+__device__ __forceinline__ float finish_g(const KParams& k, const Lane& ln, int yy, unsigned fl, bool on,
+                                          float g[9]) {
+    if (on) bc_g(k, wrap(yy, k.H), (int)ln.x, g);
+    const float s = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+    const bool ghost_row = k.y_walls && (yy < 0 || yy >= k.H);
+    return ghost_row ? k.c.psi_wall : (!ln.inside ? ln.ghost : ((fl & kSolid) ? k.c.psi_wall : s));
+}
+
+// psi of row yy of the column beside the strip (lanes 0 and 63), flags fl.
+__device__ __forceinline__ float psi_halo(const KParams& k, rsrc_t src, const Lane& lh, int yy, unsigned fl) {
+    float g[9];
+    const bool on = g_on(k, lh, yy, fl);
+    pull(src, k, 9, rows_of(k, wrap(yy, k.H)), pull_offsets(k, lh.x), fl & kLinks, on, g);
+    return finish_g(k, lh, yy, fl, on, g);
+}
+
+// Completes a PsiRow from the lane's own psi and the halo value (valid in
+// lanes 0 and 63); every lane of the wave must execute it.
+__device__ __forceinline__ PsiRow psi_row_of(float pcen, float halo, int lane) {
+    const float up = __shfl_up(pcen, 1, 64);    // lane i <- lane i - 1
+    const float dn = __shfl_down(pcen, 1, 64);  // lane i <- lane i + 1
+    PsiRow p;
+    p.c = pcen;
+    p.l = (lane == 0) ? halo : up;
+    p.r = (lane == 63) ? halo : dn;
+    return p;
 }
 
 // MODE 0: one iteration (collide and store the next post-collision state).
 // MODE 1: moments only - fields psi/rho/ux/uy, FP64 partial sums, and (if
 //         b.dst) the post-stream populations; the state is not advanced.
-// A workgroup of NT = 64*WX lanes owns the strip of columns
-// [strip*(NT-2), strip*(NT-2) + NT-2) and rows [chunk*rows, ...); lanes 0
-// and NT-1 only supply psi of the columns beside the strip.
-template <int WX, int MODE>
-__global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
-    constexpr int NT = 64 * WX;
-    constexpr int SW = NT - 2;
-    __shared__ float ring[8][NT];
-    const int t = threadIdx.x;
-    const int strip = blockIdx.x % k.nstrips;
-    const int chunk = blockIdx.x / k.nstrips;
-    const int xcol = strip * SW - 1 + t;
-    const bool interior = (t >= 1) && (t <= SW) && (xcol < k.W);
+// Workgroups of WPB waves cover WPB adjacent strips of one chunk of k.rows
+// rows.  Blocks are numbered so that each XCD (workgroups are dealt
+// round-robin over the eight XCDs) gets a contiguous band of chunks:
+// neighbouring strips and the halo rows of neighbouring chunks then share
+// that XCD's L2.  The FP64 partial sums of MODE 1 are per wave, at
+// partials[3 * (WPB * L + wave)].
+template <int WPB, int MODE>
+__global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
+    const int L = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+    const int chunk = L / k.nsb;
+    const int strip = (L - chunk * k.nsb) * WPB + wave;
+    double* part = b.partials + 3 * (WPB * L + wave);
+    const int x0 = strip * 64;
+    if (x0 >= k.W) {
+        if (MODE == 1 && lane == 0) part[0] = part[1] = part[2] = 0.0;
+        return;
+    }
+    const int xcol = x0 + lane;
     const Lane ln = lane_of(k, xcol);
+    const Lane lh = lane_of(k, lane == 0 ? x0 - 1 : x0 + 64);
+    const bool hl = (lane == 0) || (lane == 63);
+    const bool interior = xcol < k.W;
     const int y0 = chunk * k.rows;
     const int y1 = min(y0 + k.rows, k.H);
 
@@ -519,26 +459,53 @@ __global__ __launch_bounds__(64 * WX) void step_kernel(KParams k, KBufs b) {
     const rsrc_t dst = make_rsrc(b.dst, b.dst ? bytes : 0u);
     double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
     const PullOff po = pull_offsets(k, ln.x);
-    // rolling flags of rows y .. y+4
-    unsigned fl[5];
-    fl[0] = flags_of(k, b, ln, y0);
-    fl[1] = flags_of(k, b, ln, y0 + 1);
-    fl[2] = flags_of(k, b, ln, y0 + 2);
-    float g0[9], gm[9];
-    ring[0][t] = psi_row(k, src, ln, po, y0 - 1, flags_of(k, b, ln, y0 - 1), gm);
-    ring[1][t] = psi_row(k, src, ln, po, y0, fl[0], g0);
-    int y = y0;
-    for (; y + 1 < y1; y += 2) {
-        march_pair<NT, MODE, true>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, g0, fl, s_rho, s_psi,
-                                   s_bad);
-        fl[0] = fl[2];
-        fl[1] = fl[3];
-        fl[2] = fl[4];
+
+    // prologue: psi of rows y0 - 1 and y0, g of row y0
+    float gc[9];
+    PsiRow pm, pc;
+    {
+        const unsigned fm = flags_of(k, b, ln, y0 - 1), fmh = flags_of(k, b, lh, y0 - 1);
+        const bool on = g_on(k, ln, y0 - 1, fm);
+        pull(src, k, 9, rows_of(k, wrap(y0 - 1, k.H)), po, fm & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, y0 - 1, fmh);
+        pm = psi_row_of(finish_g(k, ln, y0 - 1, fm, on, gc), h, lane);
     }
-    if (y < y1)
-        march_pair<NT, MODE, false>(k, b, src, dst, ring, t, ln, po, interior, y, y - y0, g0, fl, s_rho, s_psi,
-                                    s_bad);
-    if (MODE == 1) block_sum3<NT>(s_rho, s_psi, s_bad, b.partials + 3 * blockIdx.x);
+    unsigned flc = flags_of(k, b, ln, y0);
+    {
+        const unsigned fch = flags_of(k, b, lh, y0);
+        const bool on = g_on(k, ln, y0, flc);
+        pull(src, k, 9, rows_of(k, y0), po, flc & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, y0, fch);
+        pc = psi_row_of(finish_g(k, ln, y0, flc, on, gc), h, lane);
+    }
+    unsigned fln = flags_of(k, b, ln, y0 + 1);
+    unsigned flnh = flags_of(k, b, lh, y0 + 1);
+
+    for (int y = y0; y < y1; ++y) {
+        const unsigned fl2 = flags_of(k, b, ln, y + 2);
+        const unsigned fl2h = flags_of(k, b, lh, y + 2);
+        float gn[9], f[9];
+        const bool gn_on = g_on(k, ln, y + 1, fln);
+        pull(src, k, 9, rows_of(k, wrap(y + 1, k.H)), po, fln & kLinks, gn_on, gn);
+        const bool f_on = interior && !((flc & kSolid) && k.inert);
+        pull(src, k, 0, rows_of(k, y), po, flc & kLinks, f_on, f);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, y + 1, flnh);
+        const PsiRow pn = psi_row_of(finish_g(k, ln, y + 1, fln, gn_on, gn), h, lane);
+        const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, pm.c, pn.r, pn.l, pm.l, pm.r};
+        finish_node<MODE>(k, b, src, dst, ln, po, y, interior, nb9, f, gc, interior ? flc : kSolid, s_rho, s_psi,
+                          s_bad);
+        pm = pc;
+        pc = pn;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gc[i] = gn[i];
+        flc = fln;
+        fln = fl2;
+        flnh = fl2h;
+    }
+    if (MODE == 1) wave_sum3(s_rho, s_psi, s_bad, part, lane);
 }
 
 // Initial state (fingering_periodic.py:90-121, or fingering.py:95-127 with
@@ -590,26 +557,10 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             fdn::equilibria(m, k.c, f, g);
             if (!PRE) fdn::collide(f, g, m, k.c);
         }
-        // An inert solid node's slots belong to its fluid neighbours (slot
-        // bounce-back below), so it must not write them.
-        if (PRE || !(solid && k.inert)) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                b.dst[i * k.Q + o] = f[i];
-                b.dst[(9 + i) * k.Q + o] = g[i];
-            }
-        }
-        if (!PRE && k.inert && !solid) {
-            // bounce-back through the solid neighbours' slots, as the step kernel does
-            const unsigned links = b.flags[o] & kSlotLinks;
-#pragma unroll
-            for (int i = 1; i < 9; ++i) {
-                if (!((links >> (i - 1)) & 1u)) continue;
-                const int ys = wrap(y - fdn::EY[i], k.H);
-                const int xs = wrap(x - fdn::EX[i], k.W);
-                b.dst[i * k.Q + ys * k.Wp + xs] = f[fdn::OPP[i]];
-                b.dst[(9 + i) * k.Q + ys * k.Wp + xs] = g[fdn::OPP[i]];
-            }
+        for (int i = 0; i < 9; ++i) {
+            b.dst[i * k.Q + o] = f[i];
+            b.dst[(9 + i) * k.Q + o] = g[i];
         }
         if (PRE) {
             b.fields[o] = psi;
@@ -695,8 +646,8 @@ struct fd_handle {
     int cur = 0, sc = 0;
     int64_t N = 0;
     bool inited = false;
-    int wx = 4;
-    int rows = 32;
+    int wx = 4;     // waves (64-column strips) per workgroup
+    int rows = 32;  // rows per wave
     int timed_launches = 0;
     bool have_timing = false;
 };
@@ -753,9 +704,13 @@ bool overwritten(const fd_params& p, int y, int x, int i) {
 // Pack the node flags and decide whether solid nodes are inert: that needs
 // every solid -> fluid link either bounced or rewritten by a boundary
 // condition (no leak of a solid node's streamed populations) and no solid
-// node on a Zou-He column.  Then links whose upstream node is solid become
-// slot links and the rest (over-bounced links, bounce_back.py:62-86) stay
-// explicit; otherwise every link is explicit.
+// node on a Zou-He column.  The link bits of a node are its bounce-back
+// links (bounce_back.py:25-167, over-bounced links included) plus, on the
+// wall rows, the three directions the wall reflects (fingering.py:432-451):
+// both pull the node's own opposite population.
+constexpr unsigned kWallLoLinks = (1u << 1) | (1u << 4) | (1u << 5);  // directions 2, 5, 6
+constexpr unsigned kWallHiLinks = (1u << 3) | (1u << 6) | (1u << 7);  // directions 4, 7, 8
+
 bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, int Wp, std::vector<uint32_t>& fl) {
     const int H = p.H, W = p.W;
     auto sol = [&](int y, int x) { return solid[(size_t)((y + H) % H) * W + ((x + W) % W)] != 0; };
@@ -775,22 +730,13 @@ bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, 
         }
     }
     fl.assign((size_t)H * Wp, kSolid);  // padding columns: solid, never read
+    const bool walls = p.y_bc == FD_Y_WALLS;
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
-            const unsigned lk = links[(size_t)y * W + x] & FD_LINK_BITS;
-            const bool s = sol(y, x);
-            unsigned v = s ? kSolid : 0u;
-            if (!inert) {
-                v |= lk << kExplShift;
-            } else if (!s) {
-                for (int i = 1; i < 9; ++i) {
-                    if (!((lk >> (i - 1)) & 1u)) continue;
-                    if (sol(y - fdn::EY[i], x - fdn::EX[i]))
-                        v |= 1u << (i - 1);
-                    else
-                        v |= 1u << (kExplShift + i - 1);
-                }
-            }
+            unsigned v = links[(size_t)y * W + x] & FD_LINK_BITS;
+            if (walls && y == p.wall_lo) v |= kWallLoLinks;
+            if (walls && y == p.wall_hi) v |= kWallHiLinks;
+            if (sol(y, x)) v = inert ? kSolid : (v | kSolid);
             fl[(size_t)y * Wp + x] = v;
         }
     return inert;
@@ -806,8 +752,6 @@ void fill_kparams(fd_handle* h) {
     k.y_walls = p.y_bc == FD_Y_WALLS;
     k.x_periodic = p.x_bc == FD_X_PERIODIC;
     k.pressure = p.outlet == FD_OUTLET_PRESSURE;
-    k.wall_lo = p.wall_lo;
-    k.wall_hi = p.wall_hi;
     k.inert = h->inert ? 1 : 0;
     k.rho_out = (float)p.rho_out;
     k.c3 = (float)p.outlet_f3_coef;
@@ -822,25 +766,30 @@ void fill_kparams(fd_handle* h) {
     k.c.gy = (float)p.gy;
 }
 
-int strip_width(int wx) { return 64 * wx - 2; }
+// Workgroups of h->wx waves (strips of 64 columns each), h->rows rows.
+int step_grid(const fd_handle* h, KParams* kp) {
+    const int nsb = ((h->p.W + 63) / 64 + h->wx - 1) / h->wx;
+    const int nchunks = (h->p.H + h->rows - 1) / h->rows;
+    if (kp) kp->nsb = nsb;
+    return nsb * nchunks;
+}
 
 int launch_step(fd_handle* h, int mode, const KBufs& b) {
     KParams k = h->k;
     k.rows = h->rows;
-    k.nstrips = (k.W + strip_width(h->wx) - 1) / strip_width(h->wx);
-    const int nchunks = (k.H + k.rows - 1) / k.rows;
-    const dim3 grid(k.nstrips * nchunks);
+    const dim3 grid(step_grid(h, &k));
     const dim3 block(64 * h->wx);
-#define FD_LAUNCH(WX)                                                                        \
+#define FD_LAUNCH(WPB)                                                                       \
     do {                                                                                     \
         if (mode == 0)                                                                       \
-            hipLaunchKernelGGL((step_kernel<WX, 0>), grid, block, 0, h->stream, k, b);       \
+            hipLaunchKernelGGL((step_kernel<WPB, 0>), grid, block, 0, h->stream, k, b);      \
         else                                                                                 \
-            hipLaunchKernelGGL((step_kernel<WX, 1>), grid, block, 0, h->stream, k, b);       \
+            hipLaunchKernelGGL((step_kernel<WPB, 1>), grid, block, 0, h->stream, k, b);      \
     } while (0)
     switch (h->wx) {
         case 1: FD_LAUNCH(1); break;
         case 2: FD_LAUNCH(2); break;
+        case 8: FD_LAUNCH(8); break;
         default: FD_LAUNCH(4); break;
     }
 #undef FD_LAUNCH
@@ -848,10 +797,8 @@ int launch_step(fd_handle* h, int mode, const KBufs& b) {
     return FD_OK;
 }
 
-int moments_grid_blocks(const fd_handle* h) {
-    const int nstrips = (h->p.W + strip_width(h->wx) - 1) / strip_width(h->wx);
-    return nstrips * ((h->p.H + h->rows - 1) / h->rows);
-}
+// Number of FP64 partial-sum slots (one per wave) a moments launch writes.
+int moments_grid_blocks(const fd_handle* h) { return h->wx * step_grid(h, nullptr); }
 
 int ensure_partials(fd_handle* h, int n) {
     if (n <= h->npart) return FD_OK;
@@ -953,8 +900,8 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     h->inert = pack_flags(*p, solid, links, h->Wp, fl);
     fill_kparams(h);
     // A large domain favours 4-wave strips; small ones need more workgroups.
-    h->wx = p->W >= 1024 ? 4 : (p->W >= 256 ? 2 : 1);
-    h->rows = p->H >= 1024 ? 32 : 16;
+    h->wx = 4;
+    h->rows = 32;
 
     const size_t Q = h->Q, H = p->H;
     std::vector<float> up(H);
@@ -1127,7 +1074,8 @@ double fd_last_step_ms(const fd_handle* h) {
 
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows) {
     if (!h) return fail(FD_E_ARG, "handle is NULL");
-    if (waves_x != 1 && waves_x != 2 && waves_x != 4) return fail(FD_E_ARG, "waves_x must be 1, 2 or 4");
+    if (waves_x != 1 && waves_x != 2 && waves_x != 4 && waves_x != 8)
+        return fail(FD_E_ARG, "waves_x must be 1, 2, 4 or 8");
     if (rows < 1) return fail(FD_E_ARG, "rows must be >= 1");
     h->wx = waves_x;
     h->rows = rows;

@@ -101,7 +101,8 @@ class Compute:
         return float(self._lib.fd_last_step_ms(self._h))
 
     def set_tile(self, waves_x, rows):
-        """Launch shape: strips of 64*waves_x-2 columns, `rows` rows per workgroup."""
+        """Launch shape: workgroups of waves_x (1, 2, 4 or 8) waves, each wave
+        marching `rows` rows of its own 64-column strip."""
         _native.check(self._lib.fd_set_tile(self._h, int(waves_x), int(rows)))
 
     # ------------------------------------------------------------ outputs

@@ -139,7 +139,7 @@ def test_parity_rows_per_wave_sweep():
     o = oracle_for(params, mask, links)
     o.step(9)
     ref = oracle_fields(o)
-    for wx, rows in ((1, 1), (1, 3), (2, 7), (4, 16), (1, 40), (2, 64), (4, 5)):
+    for wx, rows in ((1, 1), (1, 3), (2, 7), (4, 16), (8, 5), (1, 40), (2, 64), (8, 1)):
         cm = _compute(mask, bits, params)
         cm.set_tile(wx, rows)
         cm.step(9)
