@@ -34,22 +34,25 @@
 
 namespace {
 
-// Per-node flags (uint32):
+// Per-node flags (uint16):
 //   bits 0-7   links: direction i (bit i-1) is bounced back, i.e. pulled
 //              from the node's own post-collision opposite population
 //              (half-way bounce-back, bounce_back.py:25-167); the top and
 //              bottom walls (fingering.py:432-451, 573) are links of their
 //              rows, set by the host
-//   bit 16     solid
+//   bit 8      solid
 // In the inert-solid mode (leak-free geometry) solid nodes are never read
-// and store zeros; otherwise they stream their populations on unchanged
-// (store-and-forward, fingering_periodic.py:455-466).  Either way every node
-// of a row is stored, so the 18 planes are written in whole cache lines.
+// and store zeros (or nothing, in 64-byte groups of inert solids);
+// otherwise they stream their populations on unchanged (store-and-forward,
+// fingering_periodic.py:455-466).
 constexpr unsigned kLinks = 0xFFu;
-constexpr unsigned kSolid = 1u << 16;
+constexpr unsigned kSolid = 1u << 8;
 // A vector offset beyond any descriptor's range: such buffer loads return 0
 // and such stores are dropped, without a memory access.
 constexpr unsigned kOOB = 0x7FFFFFFCu;
+// Cache policy of the population stores: non-temporal (gfx950 "nt"); the
+// next read of a stored value is one full iteration later.
+constexpr int kStoreNT = 2;
 
 struct KParams {
     int H, W, Wp;
@@ -65,7 +68,7 @@ struct KParams {
 struct KBufs {
     const float* __restrict__ src;     // post-collision populations (18 planes)
     float* __restrict__ dst;           // next post-collision populations / post-stream copy
-    const uint32_t* __restrict__ flags;
+    const uint16_t* __restrict__ flags;
     const float* __restrict__ side_in; // [4][H] P of the previous psi
     float* __restrict__ side_out;
     const float* __restrict__ uprof;   // [H] inlet/outlet velocity
@@ -331,14 +334,20 @@ __device__ __forceinline__ void finish_node(const KParams& k, const KBufs& b, rs
                 g[i] = solid ? g[i] : go[i];
             }
         }
-        // every node of the row is stored (inert solid nodes store their
-        // zeros), predicated by offset like the loads
-        const unsigned vo = on ? po.xb : kOOB;
+        // Stores are predicated by offset like the loads, and written in
+        // whole 64-byte pieces: a group of 16 lanes (64 B of one plane row)
+        // stores all of its nodes - inert solid nodes their zeros - unless
+        // all 16 are inert solids, which nothing reads.  A piece written only
+        // in part costs far more HBM time than one written whole.
+        const unsigned long long live = __ballot(!(solid && k.inert));
+        const bool st = on && ((live >> (__lane_id() & 48u)) & 0xFFFFull) != 0ull;
+        const unsigned vo = st ? po.xb : kOOB;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, (rc + i * k.Q) * 4, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, (rc + i * k.Q) * 4,
+                                                  kStoreNT);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo,
-                                                  (rc + (9 + i) * k.Q) * 4, 0);
+                                                  (rc + (9 + i) * k.Q) * 4, kStoreNT);
         }
     } else if (on) {
         const unsigned o = rc + ln.x;
@@ -631,7 +640,7 @@ struct fd_handle {
     int Wp = 0;
     size_t Q = 0;
     float* pop[2] = {nullptr, nullptr};
-    uint32_t* flags = nullptr;
+    uint16_t* flags = nullptr;
     bool inert = false;
     float* side[2] = {nullptr, nullptr};
     float* uprof = nullptr;
@@ -647,7 +656,7 @@ struct fd_handle {
     int64_t N = 0;
     bool inited = false;
     int wx = 4;     // waves (64-column strips) per workgroup
-    int rows = 32;  // rows per wave
+    int rows = 16;  // rows per wave
     int timed_launches = 0;
     bool have_timing = false;
 };
@@ -711,7 +720,7 @@ bool overwritten(const fd_params& p, int y, int x, int i) {
 constexpr unsigned kWallLoLinks = (1u << 1) | (1u << 4) | (1u << 5);  // directions 2, 5, 6
 constexpr unsigned kWallHiLinks = (1u << 3) | (1u << 6) | (1u << 7);  // directions 4, 7, 8
 
-bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, int Wp, std::vector<uint32_t>& fl) {
+bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, int Wp, std::vector<uint16_t>& fl) {
     const int H = p.H, W = p.W;
     auto sol = [&](int y, int x) { return solid[(size_t)((y + H) % H) * W + ((x + W) % W)] != 0; };
     bool inert = true;
@@ -737,7 +746,7 @@ bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, 
             if (walls && y == p.wall_lo) v |= kWallLoLinks;
             if (walls && y == p.wall_hi) v |= kWallHiLinks;
             if (sol(y, x)) v = inert ? kSolid : (v | kSolid);
-            fl[(size_t)y * Wp + x] = v;
+            fl[(size_t)y * Wp + x] = (uint16_t)v;
         }
     return inert;
 }
@@ -896,12 +905,12 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     h->device = dev;
     h->Wp = (p->W + 63) / 64 * 64;
     h->Q = (size_t)p->H * h->Wp;
-    std::vector<uint32_t> fl;
+    std::vector<uint16_t> fl;
     h->inert = pack_flags(*p, solid, links, h->Wp, fl);
     fill_kparams(h);
     // A large domain favours 4-wave strips; small ones need more workgroups.
     h->wx = 4;
-    h->rows = 32;
+    h->rows = 16;
 
     const size_t Q = h->Q, H = p->H;
     std::vector<float> up(H);
@@ -933,13 +942,13 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
         FD_HIP_H(hipMemsetAsync(h->pop[i], 0, sizeof(float) * 18 * Q, h->stream));
         FD_HIP_H(hipMemsetAsync(h->side[i], 0, sizeof(float) * 4 * H, h->stream));
     }
-    FD_HIP_H(hipMalloc(&h->flags, sizeof(uint32_t) * Q));
+    FD_HIP_H(hipMalloc(&h->flags, sizeof(uint16_t) * Q));
     FD_HIP_H(hipMalloc(&h->uprof, sizeof(float) * H));
     FD_HIP_H(hipMalloc(&h->psi0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->rho0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->fields, sizeof(float) * 4 * Q));
     FD_HIP_H(hipMalloc(&h->mass, sizeof(double) * 3));
-    FD_HIP_H(hipMemcpyAsync(h->flags, fl.data(), sizeof(uint32_t) * Q, hipMemcpyHostToDevice, h->stream));
+    FD_HIP_H(hipMemcpyAsync(h->flags, fl.data(), sizeof(uint16_t) * Q, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipMemcpyAsync(h->uprof, up.data(), sizeof(float) * H, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipStreamSynchronize(h->stream));
 #undef FD_HIP_H
