@@ -316,23 +316,20 @@ __device__ __forceinline__ void finish_node(const KParams& k, const KBufs& b, rs
             b.side_out[s * k.H + y] = dpx * mu;
             b.side_out[(s + 1) * k.H + y] = dpy * mu;
         }
-        // collided on every lane (values of solid lanes are discarded by the
-        // select), so the pulled registers are read on every path and the
-        // stores below never wait for an older store to complete
+        // Collided on every lane, so the pulled registers are read on every
+        // path and the stores below never wait for an older store.  On a
+        // solid lane the collision is made the identity - zero relaxation
+        // rates, zero force and velocity, finite moments - which streams
+        // its populations on unchanged (store-and-forward) or keeps the
+        // zeros of an inert solid.
         {
-            const fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
-            float fo[9], go[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                fo[i] = f[i];
-                go[i] = g[i];
+            fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
+            fdn::Consts<float> cl = k.c;
+            if (solid) {
+                m.rho = m.psi = m.mu = m.ux = m.uy = m.p = m.itau = m.fx = m.fy = 0.f;
+                cl.inv_tau_g = 0.f;
             }
-            fdn::collide(fo, go, m, k.c);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                f[i] = solid ? f[i] : fo[i];
-                g[i] = solid ? g[i] : go[i];
-            }
+            fdn::collide(f, g, m, cl);
         }
         // Stores are predicated by offset like the loads, and written in
         // whole 64-byte pieces: a group of 16 lanes (64 B of one plane row)
@@ -442,7 +439,10 @@ __device__ __forceinline__ PsiRow psi_row_of(float pcen, float halo, int lane) {
 // that XCD's L2.  The FP64 partial sums of MODE 1 are per wave, at
 // partials[3 * (WPB * L + wave)].
 template <int WPB, int MODE>
-__global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
+#ifndef FD_WPE
+#define FD_WPE 4
+#endif
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(FD_WPE, 8))) void step_kernel(KParams k, KBufs b) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
@@ -469,50 +469,61 @@ __global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
     double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
     const PullOff po = pull_offsets(k, ln.x);
 
-    // prologue: psi of rows y0 - 1 and y0, g of row y0
-    float gc[9];
-    PsiRow pm, pc;
-    {
-        const unsigned fm = flags_of(k, b, ln, y0 - 1), fmh = flags_of(k, b, lh, y0 - 1);
-        const bool on = g_on(k, ln, y0 - 1, fm);
-        pull(src, k, 9, rows_of(k, wrap(y0 - 1, k.H)), po, fm & kLinks, on, gc);
-        float h = 0.f;
-        if (hl) h = psi_halo(k, src, lh, y0 - 1, fmh);
-        pm = psi_row_of(finish_g(k, ln, y0 - 1, fm, on, gc), h, lane);
-    }
-    unsigned flc = flags_of(k, b, ln, y0);
-    {
-        const unsigned fch = flags_of(k, b, lh, y0);
-        const bool on = g_on(k, ln, y0, flc);
-        pull(src, k, 9, rows_of(k, y0), po, flc & kLinks, on, gc);
-        float h = 0.f;
-        if (hl) h = psi_halo(k, src, lh, y0, fch);
-        pc = psi_row_of(finish_g(k, ln, y0, flc, on, gc), h, lane);
-    }
-    unsigned fln = flags_of(k, b, ln, y0 + 1);
-    unsigned flnh = flags_of(k, b, lh, y0 + 1);
+    // Even chunks march up, odd chunks down: the two chunks beside a chunk
+    // boundary then pull its rows at the same time (both first or both
+    // last), so the second pull of those rows hits the L2.
+    const int d = (chunk & 1) ? -1 : 1;
+    const int ys = (d > 0) ? y0 : y1 - 1;
 
-    for (int y = y0; y < y1; ++y) {
-        const unsigned fl2 = flags_of(k, b, ln, y + 2);
-        const unsigned fl2h = flags_of(k, b, lh, y + 2);
-        float gn[9], f[9];
-        const bool gn_on = g_on(k, ln, y + 1, fln);
-        pull(src, k, 9, rows_of(k, wrap(y + 1, k.H)), po, fln & kLinks, gn_on, gn);
+    // prologue: psi of rows ys - d (behind) and ys, g of row ys
+    float gc[9];
+    PsiRow pb, pc;
+    {
+        const int yb = ys - d;
+        const unsigned fb = flags_of(k, b, ln, yb), fbh = flags_of(k, b, lh, yb);
+        const bool on = g_on(k, ln, yb, fb);
+        pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, yb, fbh);
+        pb = psi_row_of(finish_g(k, ln, yb, fb, on, gc), h, lane);
+    }
+    unsigned flc = flags_of(k, b, ln, ys);
+    {
+        const unsigned fch = flags_of(k, b, lh, ys);
+        const bool on = g_on(k, ln, ys, flc);
+        pull(src, k, 9, rows_of(k, ys), po, flc & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, ys, fch);
+        pc = psi_row_of(finish_g(k, ln, ys, flc, on, gc), h, lane);
+    }
+    unsigned fla = flags_of(k, b, ln, ys + d);
+    unsigned flah = flags_of(k, b, lh, ys + d);
+
+    for (int j = 0, y = ys; j < y1 - y0; ++j, y += d) {
+        // row y + d is the one ahead; its psi completes the stencil of row y
+        const int ya = y + d;
+        const unsigned fl2 = flags_of(k, b, ln, ya + d);
+        const unsigned fl2h = flags_of(k, b, lh, ya + d);
+        float ga[9], f[9];
+        const bool ga_on = g_on(k, ln, ya, fla);
+        pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
         const bool f_on = interior && !((flc & kSolid) && k.inert);
         pull(src, k, 0, rows_of(k, y), po, flc & kLinks, f_on, f);
         float h = 0.f;
-        if (hl) h = psi_halo(k, src, lh, y + 1, flnh);
-        const PsiRow pn = psi_row_of(finish_g(k, ln, y + 1, fln, gn_on, gn), h, lane);
-        const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, pm.c, pn.r, pn.l, pm.l, pm.r};
+        if (hl) h = psi_halo(k, src, lh, ya, flah);
+        const PsiRow pa = psi_row_of(finish_g(k, ln, ya, fla, ga_on, ga), h, lane);
+        const PsiRow& pn = (d > 0) ? pa : pb;  // row y + 1
+        const PsiRow& ps = (d > 0) ? pb : pa;  // row y - 1
+        const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, ps.c, pn.r, pn.l, ps.l, ps.r};
         finish_node<MODE>(k, b, src, dst, ln, po, y, interior, nb9, f, gc, interior ? flc : kSolid, s_rho, s_psi,
                           s_bad);
-        pm = pc;
-        pc = pn;
+        pb = pc;
+        pc = pa;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) gc[i] = gn[i];
-        flc = fln;
-        fln = fl2;
-        flnh = fl2h;
+        for (int i = 0; i < 9; ++i) gc[i] = ga[i];
+        flc = fla;
+        fla = fl2;
+        flah = fl2h;
     }
     if (MODE == 1) wave_sum3(s_rho, s_psi, s_bad, part, lane);
 }
