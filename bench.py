@@ -74,6 +74,17 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
+def copy_probe(reps=10):
+    """Best 1 GiB device-to-device streaming copy (read + write bytes / time)."""
+    import ctypes
+    from fingering_dynamics_amd import _native
+    lib = _native.load()
+    ms = ctypes.c_double()
+    nbytes = 1 << 30
+    _native.check(lib.fd_copy_probe(nbytes, reps, ctypes.byref(ms)))
+    return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
 CPU_WINDOW = (512, 1024)              # rows, columns of the CPU-baseline sample
 
 
@@ -111,21 +122,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--cpu-steps", type=int, default=15,
+    ap.add_argument("--cpu-steps", type=int, default=100,
                     help="iterations of the CPU baseline sample (0 skips it)")
     ap.add_argument("--tile", default="", help="launch shape WX,ROWS (default: the library's choice)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
     args = ap.parse_args()
     if args.hbm_probe:
-        import ctypes
-        from fingering_dynamics_amd import _native
-        lib = _native.load()
-        ms = ctypes.c_double()
-        nbytes = 1 << 30
-        _native.check(lib.fd_copy_probe(nbytes, 20, ctypes.byref(ms)))
-        print(json.dumps({"probe": "plain copy", "bytes_read": nbytes, "bytes_written": nbytes,
-                          "ms_per_copy": ms.value, "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        ms, gbps = copy_probe()
+        print(json.dumps({"probe": "streaming copy (fd_copy_probe)", "bytes_read": 1 << 30,
+                          "bytes_written": 1 << 30, "ms_per_copy": ms, "GBps": gbps}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -154,21 +160,27 @@ def main():
     kernel_ms = cm.last_step_ms()
     s_rho, s_psi, bad = cm.mass()
     cm.close()
+    _, copy_gbps = copy_probe()
 
     ms_per_step = dt * 1e3 / args.steps
     mlups = H * W * args.steps / dt / 1e6
     algo_bytes = BYTES_PER_FLUID_UPDATE * n_fluid
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    traffic = traffic_from_profiles()
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1) if achieved else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": traffic_from_profiles(),
+        "traffic": traffic,
         "kernel_ms": round(kernel_ms, 5),
         "algorithmic_bytes_per_launch": algo_bytes,
         "fluid_nodes": n_fluid,
+        # measured HBM bytes (rocprofv3, profiles/) over this run's kernel time
+        "hbm_rate_GBps": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic and kernel_ms > 0 else None,
+        # the best streaming copy this device reached in the same process
+        "copy_GBps": round(copy_gbps, 1),
     }
     cpu = cpu_baseline(mask, bits, params, args.cpu_steps) if args.cpu_steps > 0 else None
     out = {

@@ -602,17 +602,15 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
     if (PRE) block_sum3<256>(s_rho, s_psi, s_bad, b.partials + 3 * (blockIdx.y * gridDim.x + blockIdx.x));
 }
 
-// Plain streaming copy, the HBM yardstick: each lane moves four
-// independent 16-byte vectors per trip (n must be a multiple of 4 * grid).
+// Streaming copy, the HBM yardstick: grid-stride 16-byte non-temporal
+// loads and stores (the fastest plain copy measured on MI355X).
 __global__ __launch_bounds__(256) void fd_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b,
                                                       int64_t n) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 3 * stride < n; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        b[i] = v0;
-        b[i + stride] = v1;
-        b[i + 2 * stride] = v2;
-        b[i + 3 * stride] = v3;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(a) + i);
+        __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(b) + i);
     }
 }
 
@@ -1035,15 +1033,15 @@ int64_t fd_steps_done(const fd_handle* h) { return h ? h->N : -1; }
 int fd_solid_inert(const fd_handle* h) { return h ? (h->inert ? 1 : 0) : -1; }
 
 int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy) {
-    // whole trips of 8192 x 256 lanes x 4 vectors x 16 B (32 MiB)
-    const int64_t trip = (int64_t)8192 * 256 * 4;
-    if (nbytes < trip * 16 || reps < 1 || !ms_per_copy) return fail(FD_E_ARG, "copy probe needs >= 32 MiB");
-    const int64_t n = nbytes / 16 / trip * trip;
+    if (nbytes < ((int64_t)32 << 20) || reps < 1 || !ms_per_copy) return fail(FD_E_ARG, "copy probe needs >= 32 MiB");
+    const int64_t n = nbytes / 16;
+    // the best of three launch shapes (few long-lived waves to one lane per vector)
+    const int grids[3] = {1024, 4096, (int)std::min<int64_t>((n + 255) / 256, 1 << 20)};
     float4 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = FD_OK;
-    float ms = 0.f;
+    double best = -1.0;
 #define FD_HIP_P(call)                                                                     \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \
@@ -1058,15 +1056,19 @@ int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy) {
     FD_HIP_P(hipMalloc(&a, n * 16));
     FD_HIP_P(hipMalloc(&b, n * 16));
     FD_HIP_P(hipMemsetAsync(a, 0, n * 16, st));
-    hipLaunchKernelGGL(fd_copy_kernel, dim3(8192), dim3(256), 0, st, a, b, n);  // warm-up
-    FD_HIP_P(hipGetLastError());
-    FD_HIP_P(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(fd_copy_kernel, dim3(8192), dim3(256), 0, st, a, b, n);
-    FD_HIP_P(hipGetLastError());
-    FD_HIP_P(hipEventRecord(e1, st));
-    FD_HIP_P(hipEventSynchronize(e1));
-    FD_HIP_P(hipEventElapsedTime(&ms, e0, e1));
-    *ms_per_copy = (double)ms / reps;
+    for (int gi = 0; gi < 3; ++gi) {
+        float ms = 0.f;
+        hipLaunchKernelGGL(fd_copy_kernel, dim3(grids[gi]), dim3(256), 0, st, a, b, n);  // warm-up
+        FD_HIP_P(hipGetLastError());
+        FD_HIP_P(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(fd_copy_kernel, dim3(grids[gi]), dim3(256), 0, st, a, b, n);
+        FD_HIP_P(hipGetLastError());
+        FD_HIP_P(hipEventRecord(e1, st));
+        FD_HIP_P(hipEventSynchronize(e1));
+        FD_HIP_P(hipEventElapsedTime(&ms, e0, e1));
+        if (best < 0.0 || (double)ms / reps < best) best = (double)ms / reps;
+    }
+    *ms_per_copy = best;
 #undef FD_HIP_P
 done:
     if (st) (void)hipStreamSynchronize(st);

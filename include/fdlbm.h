@@ -139,26 +139,29 @@ int fd_synchronize(fd_handle* h);
  * with HIP events on the handle's stream; < 0 if unavailable. */
 double fd_last_step_ms(const fd_handle* h);
 
-/* Kernel launch shape: each workgroup owns a strip of 64*waves_x - 2
- * columns (waves_x in {1, 2, 4}) and walks `rows` rows (>= 1).  Used by
- * the tile autotuner; fd_create picks a default from the domain size.  No
- * reference counterpart (a launch-shape control of this implementation). */
+/* Kernel launch shape: workgroups of waves_x waves (1, 2, 4 or 8), each
+ * wave marching `rows` rows (>= 1) of its own 64-column strip.  The default
+ * is 4 x 16.  No reference counterpart (a launch-shape control of this
+ * implementation). */
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
 
 /* 1 if the handle runs with inert solid nodes, 0 if solid nodes stream
  * their populations on (store-and-forward, fingering_periodic.py:455-466),
  * -1 for NULL.  Inert needs a leak-free geometry: every solid -> fluid link
  * bounced (or rewritten by a boundary condition) and no solid node on a
- * Zou-He column.  Solid nodes are then neither read nor written, bounced
- * links cost no extra load, and populations reported on solid nodes are 0.
+ * Zou-He column.  Solid nodes are then never read and hold zeros, and
+ * populations reported on solid nodes are 0.  In both modes a bounced link
+ * costs no extra load (the pull reads the node's own opposite population).
  * No reference counterpart (a property of this implementation). */
 int fd_solid_inert(const fd_handle* h);
 
-/* HBM calibration: copies nbytes (rounded down to 16 B) between two device
- * buffers the library allocates, `reps` times, with a plain streaming copy
- * kernel (fd_copy_kernel), and returns the average milliseconds per copy.
- * Used to price the roofline and to scale rocprofv3 FETCH_SIZE / WRITE_SIZE
- * counters.  No reference counterpart (a measurement aid). */
+/* HBM calibration: copies nbytes (>= 32 MiB, rounded down to 16 B) between
+ * two device buffers the library allocates, `reps` times per launch shape,
+ * with a streaming copy kernel (fd_copy_kernel: 16-byte non-temporal loads
+ * and stores) at three grid sizes, and returns the best average milliseconds
+ * per copy - the streaming rate this device reaches in practice.  Also the
+ * yardstick that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters.  No
+ * reference counterpart (a measurement aid). */
 int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy);
 
 /* Release every device buffer of the handle.  NULL is ignored. */
