@@ -36,6 +36,7 @@ EXPORTS = {
     "fd_steps_done": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_solid_inert": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_get_fields": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "fd_get_fields_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 4),
     "fd_get_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_phase_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fd_state_bytes": (ctypes.c_int64, [ctypes.c_void_p]),

@@ -58,7 +58,8 @@ struct KParams {
     int H, W, Wp;
     int Q;          // plane stride in floats (H * Wp)
     int rows;       // rows per wave
-    int nsb;        // workgroups across x (each covers WPB strips of 64 columns)
+    int nsb;        // workgroups across x in this launch (each covers WPB strips of 64 columns)
+    int sb0;        // first workgroup column of this launch (0 unless a column window)
     int y_walls, x_periodic, pressure;
     int inert;      // solid nodes are never read (leak-free geometry)
     float rho_out, c3;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(FD_WPE
     const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
     const int L = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
     const int chunk = L / k.nsb;
-    const int strip = (L - chunk * k.nsb) * WPB + wave;
+    const int strip = (k.sb0 + L - chunk * k.nsb) * WPB + wave;
     double* part = b.partials + 3 * (WPB * L + wave);
     const int x0 = strip * 64;
     if (x0 >= k.W) {
@@ -784,18 +785,25 @@ void fill_kparams(fd_handle* h) {
     k.c.gy = (float)p.gy;
 }
 
-// Workgroups of h->wx waves (strips of 64 columns each), h->rows rows.
-int step_grid(const fd_handle* h, KParams* kp) {
-    const int nsb = ((h->p.W + 63) / 64 + h->wx - 1) / h->wx;
+// Workgroups of h->wx waves (strips of 64 columns each), h->rows rows;
+// workgroup columns [sb_lo, sb_hi) of the domain (sb_hi < 0: all).
+int all_sb(const fd_handle* h) { return ((h->p.W + 63) / 64 + h->wx - 1) / h->wx; }
+
+int step_grid(const fd_handle* h, KParams* kp, int sb_lo = 0, int sb_hi = -1) {
+    if (sb_hi < 0) sb_hi = all_sb(h);
+    const int nsb = sb_hi - sb_lo;
     const int nchunks = (h->p.H + h->rows - 1) / h->rows;
-    if (kp) kp->nsb = nsb;
+    if (kp) {
+        kp->nsb = nsb;
+        kp->sb0 = sb_lo;
+    }
     return nsb * nchunks;
 }
 
-int launch_step(fd_handle* h, int mode, const KBufs& b) {
+int launch_step(fd_handle* h, int mode, const KBufs& b, int sb_lo = 0, int sb_hi = -1) {
     KParams k = h->k;
     k.rows = h->rows;
-    const dim3 grid(step_grid(h, &k));
+    const dim3 grid(step_grid(h, &k, sb_lo, sb_hi));
     const dim3 block(64 * h->wx);
 #define FD_LAUNCH(WPB)                                                                       \
     do {                                                                                     \
@@ -860,6 +868,24 @@ int compute_fields(fd_handle* h, float* pops) {
     hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, h->stream, h->partials, nblk, h->mass);
     FD_HIP(hipGetLastError());
     return FD_OK;
+}
+
+// Fields of columns [x0, x1) only (the moments pass over the strips that
+// hold them; no FP64 sums).  At N = 0 the full initial fields are used.
+int compute_fields_window(fd_handle* h, int x0, int x1) {
+    if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
+    if (h->N == 0) return compute_fields(h, nullptr);
+    const int sb_lo = (x0 / 64) / h->wx, sb_hi = ((x1 - 1) / 64) / h->wx + 1;
+    int rc = ensure_partials(h, h->wx * step_grid(h, nullptr, sb_lo, sb_hi));
+    if (rc) return rc;
+    KBufs b{};
+    b.flags = h->flags;
+    b.uprof = h->uprof;
+    b.fields = h->fields;
+    b.partials = h->partials;
+    b.src = h->pop[h->cur];
+    b.side_in = h->side[h->sc];
+    return launch_step(h, 1, b, sb_lo, sb_hi);
 }
 
 int copy_plane_out(fd_handle* h, float* dst, const float* plane) {
@@ -1112,6 +1138,22 @@ int fd_get_fields(fd_handle* h, float* psi, float* rho, float* ux, float* uy) {
     float* outs[4] = {psi, rho, ux, uy};
     for (int i = 0; i < 4; ++i)
         if (outs[i] && (rc = copy_plane_out(h, outs[i], h->fields + i * h->Q))) return rc;
+    FD_HIP(hipStreamSynchronize(h->stream));
+    return FD_OK;
+}
+
+int fd_get_fields_window(fd_handle* h, int32_t x0, int32_t x1, float* psi, float* rho, float* ux, float* uy) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    if (x0 < 0 || x1 > h->p.W || x0 >= x1) return fail(FD_E_ARG, "need 0 <= x0 < x1 <= W");
+    FD_HIP(hipSetDevice(h->device));
+    int rc = compute_fields_window(h, x0, x1);
+    if (rc) return rc;
+    const size_t w = (size_t)(x1 - x0);
+    float* outs[4] = {psi, rho, ux, uy};
+    for (int i = 0; i < 4; ++i)
+        if (outs[i])
+            FD_HIP(hipMemcpy2DAsync(outs[i], sizeof(float) * w, h->fields + i * h->Q + x0, sizeof(float) * h->Wp,
+                                    sizeof(float) * w, h->p.H, hipMemcpyDeviceToHost, h->stream));
     FD_HIP(hipStreamSynchronize(h->stream));
     return FD_OK;
 }

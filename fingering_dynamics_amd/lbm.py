@@ -62,6 +62,7 @@ class Compute:
         bits = np.ascontiguousarray(links, dtype=np.uint8)
         if bits.shape != (H, W):
             raise ValueError("links must be uint8[H][W]")
+        self.links = bits
         self._fdp = _to_fd_params(params)
         h = _native.ctypes.c_void_p()
         _native.check(self._lib.fd_create(_native.ctypes.byref(self._fdp), _native.ptr(solid),
@@ -113,6 +114,14 @@ class Compute:
             _native.check(self._lib.fd_get_fields(self._h, *[_native.ptr(a) for a in out]))
             self._cache = tuple(out)
         return self._cache
+
+    def fields_window(self, x0, x1):
+        """(psi, rho, ux, uy) of columns [x0, x1) as float32 [H][x1-x0]; only
+        the strips holding the window are evaluated."""
+        w = int(x1) - int(x0)
+        out = [np.empty((self.H, w), dtype=np.float32) for _ in range(4)]
+        _native.check(self._lib.fd_get_fields_window(self._h, int(x0), int(x1), *[_native.ptr(a) for a in out]))
+        return tuple(out)
 
     @property
     def psi(self):

@@ -112,6 +112,15 @@ int64_t fd_steps_done(const fd_handle* h);
  * psi_wall there).  Synchronises the handle's stream. */
 int fd_get_fields(fd_handle* h, float* psi, float* rho, float* ux, float* uy);
 
+/* The same fields restricted to columns [x0, x1), float[H*(x1-x0)] each
+ * (row-major); any pointer may be NULL.  Only the 64-column strips holding
+ * the window are evaluated, so reading the outlet column to detect
+ * breakthrough (any fluid node of column W-1 with psi > 0) costs a small
+ * fraction of an iteration.  Partial read of cm.psi / cm.ux ... of
+ * fingering_periodic.py:486-487; no separate reference counterpart.
+ * Synchronises the handle's stream. */
+int fd_get_fields_window(fd_handle* h, int32_t x0, int32_t x1, float* psi, float* rho, float* ux, float* uy);
+
 /* Post-stream populations f, g float[9*H*W] after the last completed
  * iteration (cm.f, cm.g).  Either pointer may be NULL.  Synchronises. */
 int fd_get_populations(fd_handle* h, float* f, float* g);
