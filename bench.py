@@ -130,6 +130,50 @@ def cpu_baseline(mask, bits, params, steps):
     }
 
 
+def ensemble_bench(n, steps, warmup):
+    """Aggregate MLUPS of n independent domains (the seeded many-domain
+    workload of fingering_dynamics_amd.ensemble.domain_spec) advanced by one
+    graph-replayed launch per iteration, against one launch per domain per
+    iteration on the same stream.  Prints one JSON line."""
+    from fingering_dynamics_amd.ensemble import Ensemble, domain_spec, build_domain
+    from fingering_dynamics_amd.lbm import Compute
+    t0 = time.perf_counter()
+    cs = [Compute(*build_domain(domain_spec(k))) for k in range(n)]
+    setup_s = time.perf_counter() - t0
+    ens = Ensemble(cs)
+    tune = ens.autotune(steps=10)
+    ens.step(warmup)
+    ens.synchronize()
+    nodes = ens.nodes
+    res = {}
+    for mode in ("graph", "eager", "sequential"):
+        k = steps if mode != "sequential" else max(steps // 8, 10)
+        ens.use_graph(mode == "graph")
+        t1 = time.perf_counter()
+        (ens.step_sequential if mode == "sequential" else ens.step)(k)
+        ens.synchronize()
+        wall = time.perf_counter() - t1
+        res[mode] = {"steps": k, "ms_per_step": round(ens.last_step_ms(), 4),
+                     "wall_ms_per_step": round(wall * 1e3 / k, 4),
+                     "mlups": round(nodes * k / wall / 1e6, 1)}
+    bad = [c.mass()[2] for c in cs]
+    out = {
+        "metric": "aggregate MLUPS of %d independent domains, one launch per iteration vs one per domain" % n,
+        "value": res["graph"]["mlups"], "unit": "MLUPS", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "fp32", "data": "synthetic: seeded squares/circles, sizes 256..1024 x 128..512",
+        "config": {"workload": "many-domain ensemble", "domains": n, "nodes": nodes},
+        "batched_graph": res["graph"], "batched_eager": res["eager"], "sequential": res["sequential"],
+        "speedup_vs_sequential": round(res["graph"]["mlups"] / res["sequential"]["mlups"], 2),
+        "autotune_ms_per_step": {"%dx%d" % t: round(v, 4) for t, v in tune.items()},
+        "tile": "%dx%d" % ens.tile, "setup_s": round(setup_s, 1),
+        "domains_non_finite": int(sum(1 for v in bad if v)), "iterations_run": int(cs[0].t),
+    }
+    ens.close()
+    for c in cs:
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,11 +184,16 @@ def main():
     ap.add_argument("--tile", default="", help="launch shape WX,ROWS (default: the library's choice)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="instead: the many-domain workload with N domains, batched vs sequential launches")
     args = ap.parse_args()
     if args.hbm_probe:
         ms, gbps = copy_probe()
         print(json.dumps({"probe": "streaming copy (fd_copy_probe)", "bytes_read": 1 << 30,
                           "bytes_written": 1 << 30, "ms_per_copy": ms, "GBps": gbps}), flush=True)
+        return
+    if args.ensemble:
+        ensemble_bench(args.ensemble, args.steps if args.steps != 20000 else 2000, args.warmup if args.warmup != 500 else 20)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

@@ -47,6 +47,14 @@ EXPORTS = {
     "fd_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fd_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "fd_destroy": (None, [ctypes.c_void_p]),
+    "fd_ens_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "fd_ens_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fd_ens_step_sequential": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fd_ens_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "fd_ens_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fd_ens_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fd_ens_last_step_ms": (ctypes.c_double, [ctypes.c_void_p]),
+    "fd_ens_destroy": (None, [ctypes.c_void_p]),
     "fd_last_error": (ctypes.c_char_p, []),
 }
 

@@ -439,15 +439,19 @@ __device__ __forceinline__ PsiRow psi_row_of(float pcen, float halo, int lane) {
 // neighbouring strips and the halo rows of neighbouring chunks then share
 // that XCD's L2.  The FP64 partial sums of MODE 1 are per wave, at
 // partials[3 * (WPB * L + wave)].
+// XCD-banded logical index of this workgroup: workgroups are dealt
+// round-robin over the eight XCDs, so XCD j runs blocks j, j+8, ...; they
+// are renumbered to one contiguous range per XCD.
+__device__ __forceinline__ int banded_block() {
+    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
+    return xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+}
+
+// The march of logical workgroup L of one domain (see step_kernel).
 template <int WPB, int MODE>
-#ifndef FD_WPE
-#define FD_WPE 4
-#endif
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(FD_WPE, 8))) void step_kernel(KParams k, KBufs b) {
+__device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, int L) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
-    const int L = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
     const int chunk = L / k.nsb;
     const int strip = (k.sb0 + L - chunk * k.nsb) * WPB + wave;
     double* part = b.partials + 3 * (WPB * L + wave);
@@ -527,6 +531,50 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(FD_WPE
         flah = fl2h;
     }
     if (MODE == 1) wave_sum3(s_rho, s_psi, s_bad, part, lane);
+}
+
+template <int WPB, int MODE>
+__global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
+    march_block<WPB, MODE>(k, b, banded_block());
+}
+
+// One domain of a batched launch (an ensemble): its constants, flags,
+// inlet profile, both population and side-buffer sets, the parity of its
+// handle when the table was built, and its first logical workgroup.
+struct EnsDomain {
+    KParams k;
+    const uint16_t* flags;
+    const float* uprof;
+    float* pop[2];
+    float* side[2];
+    int cur0, sc0;
+    int blk0;
+    int pad;
+};
+
+// Iteration of every domain of an ensemble in one launch.  Logical
+// workgroups (XCD-banded as in step_kernel) are laid out domain after
+// domain; a workgroup finds its domain by binary search over blk0 (scalar
+// loads: the index is uniform) and marches it with that domain's constants
+// and the buffer set selected by `parity` (launches alternate 0, 1, ...).
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void ens_kernel(const EnsDomain* __restrict__ doms, int ndom, int parity) {
+    const int L = banded_block();
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (doms[mid].blk0 <= L) lo = mid; else hi = mid - 1;
+    }
+    const EnsDomain& e = doms[lo];
+    const int pc = e.cur0 ^ parity, ps = e.sc0 ^ parity;
+    KBufs b{};
+    b.src = e.pop[pc];
+    b.dst = e.pop[pc ^ 1];
+    b.flags = e.flags;
+    b.side_in = e.side[ps];
+    b.side_out = e.side[ps ^ 1];
+    b.uprof = e.uprof;
+    march_block<WPB, 0>(e.k, b, L - e.blk0);
 }
 
 // Initial state (fingering_periodic.py:90-121, or fingering.py:95-127 with
@@ -662,6 +710,7 @@ struct fd_handle {
     int npart = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evs = nullptr;  // orders the handle's stream against an ensemble's
     int cur = 0, sc = 0;
     int64_t N = 0;
     bool inited = false;
@@ -800,17 +849,19 @@ int step_grid(const fd_handle* h, KParams* kp, int sb_lo = 0, int sb_hi = -1) {
     return nsb * nchunks;
 }
 
-int launch_step(fd_handle* h, int mode, const KBufs& b, int sb_lo = 0, int sb_hi = -1) {
+int launch_step(fd_handle* h, int mode, const KBufs& b, int sb_lo = 0, int sb_hi = -1,
+                hipStream_t stream = nullptr) {
     KParams k = h->k;
     k.rows = h->rows;
     const dim3 grid(step_grid(h, &k, sb_lo, sb_hi));
     const dim3 block(64 * h->wx);
+    hipStream_t st = stream ? stream : h->stream;
 #define FD_LAUNCH(WPB)                                                                       \
     do {                                                                                     \
         if (mode == 0)                                                                       \
-            hipLaunchKernelGGL((step_kernel<WPB, 0>), grid, block, 0, h->stream, k, b);      \
+            hipLaunchKernelGGL((step_kernel<WPB, 0>), grid, block, 0, st, k, b);             \
         else                                                                                 \
-            hipLaunchKernelGGL((step_kernel<WPB, 1>), grid, block, 0, h->stream, k, b);      \
+            hipLaunchKernelGGL((step_kernel<WPB, 1>), grid, block, 0, st, k, b);             \
     } while (0)
     switch (h->wx) {
         case 1: FD_LAUNCH(1); break;
@@ -917,6 +968,7 @@ void fd_destroy(fd_handle* h) {
     if (h->mass) (void)hipFree(h->mass);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->evs) (void)hipEventDestroy(h->evs);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -971,6 +1023,7 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     FD_HIP_H(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     FD_HIP_H(hipEventCreate(&h->ev0));
     FD_HIP_H(hipEventCreate(&h->ev1));
+    FD_HIP_H(hipEventCreateWithFlags(&h->evs, hipEventDisableTiming));
     for (int i = 0; i < 2; ++i) {
         FD_HIP_H(hipMalloc(&h->pop[i], sizeof(float) * 18 * Q));
         FD_HIP_H(hipMalloc(&h->side[i], sizeof(float) * 4 * H));
@@ -1230,6 +1283,265 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
     h->inited = true;
     h->have_timing = false;
     return FD_OK;
+}
+
+
+// ------------------------------------------------------------------ ensemble
+}  // extern "C"
+
+struct fd_ensemble {
+    std::vector<fd_handle*> hs;
+    std::vector<int> cur0;           // handle parities when the table was built
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evd = nullptr;
+    EnsDomain* table = nullptr;
+    int nblocks = 0;
+    int wx = 4, rows = 16;
+    bool table_ok = false;
+    int64_t flips = 0;               // launches since the table was built
+    bool use_graph = true;
+    hipGraphExec_t exec = nullptr;
+    int timed_launches = 0;
+    bool have_timing = false;
+};
+
+namespace {
+
+int ens_build(fd_ensemble* e) {
+    if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
+    std::vector<EnsDomain> t(e->hs.size());
+    int64_t blk = 0;
+    for (size_t i = 0; i < e->hs.size(); ++i) {
+        fd_handle* h = e->hs[i];
+        EnsDomain& d = t[i];
+        std::memset(&d, 0, sizeof d);
+        d.k = h->k;
+        d.k.rows = e->rows;
+        d.k.nsb = ((h->p.W + 63) / 64 + e->wx - 1) / e->wx;
+        d.k.sb0 = 0;
+        d.flags = h->flags;
+        d.uprof = h->uprof;
+        d.pop[0] = h->pop[0];
+        d.pop[1] = h->pop[1];
+        d.side[0] = h->side[0];
+        d.side[1] = h->side[1];
+        d.cur0 = h->cur;
+        d.sc0 = h->sc;
+        d.blk0 = (int)blk;
+        e->cur0[i] = h->cur;
+        blk += (int64_t)d.k.nsb * ((h->p.H + e->rows - 1) / e->rows);
+    }
+    if (blk >= ((int64_t)1 << 31)) return fail(FD_E_ARG, "ensemble too large for one launch");
+    FD_HIP(hipMemcpyAsync(e->table, t.data(), sizeof(EnsDomain) * t.size(), hipMemcpyHostToDevice, e->stream));
+    FD_HIP(hipStreamSynchronize(e->stream));
+    e->nblocks = (int)blk;
+    e->flips = 0;
+    e->table_ok = true;
+    return FD_OK;
+}
+
+int ens_launch(fd_ensemble* e, int parity) {
+    const dim3 grid(e->nblocks), block(64 * e->wx);
+    const int n = (int)e->hs.size();
+    switch (e->wx) {
+        case 1: hipLaunchKernelGGL((ens_kernel<1>), grid, block, 0, e->stream, e->table, n, parity); break;
+        case 2: hipLaunchKernelGGL((ens_kernel<2>), grid, block, 0, e->stream, e->table, n, parity); break;
+        case 8: hipLaunchKernelGGL((ens_kernel<8>), grid, block, 0, e->stream, e->table, n, parity); break;
+        default: hipLaunchKernelGGL((ens_kernel<4>), grid, block, 0, e->stream, e->table, n, parity); break;
+    }
+    FD_HIP(hipGetLastError());
+    return FD_OK;
+}
+
+// Two launches (parity 0, then 1) captured once and replayed: the buffer
+// sets alternate, so a pair returns every pointer to where it started.
+int ens_graph(fd_ensemble* e) {
+    if (e->exec) return FD_OK;
+    hipGraph_t g = nullptr;
+    FD_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ens_launch(e, 0);
+    if (!rc) rc = ens_launch(e, 1);
+    const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(FD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        e->exec = nullptr;
+        return fail(FD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return FD_OK;
+}
+
+int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
+    if (!e) return fail(FD_E_ARG, "ensemble is NULL");
+    if (nsteps < 0) return fail(FD_E_ARG, "nsteps < 0");
+    const int64_t N = e->hs[0]->N;
+    for (fd_handle* h : e->hs) {
+        if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called on every handle");
+        if (h->N != N) return fail(FD_E_STATE, "the handles of an ensemble must be at the same iteration");
+    }
+    FD_HIP(hipSetDevice(e->device));
+    const int64_t nl = std::max<int64_t>(N + nsteps - 1, 0) - std::max<int64_t>(N - 1, 0);
+    if (nl > 0) {
+        int rc = FD_OK;
+        bool consistent = e->table_ok;
+        for (size_t i = 0; consistent && i < e->hs.size(); ++i)
+            consistent = e->hs[i]->cur == (e->cur0[i] ^ (int)(e->flips & 1));
+        if (!consistent && (rc = ens_build(e))) return rc;
+        // the handles' own queued work comes first
+        for (fd_handle* h : e->hs) {
+            FD_HIP(hipEventRecord(h->evs, h->stream));
+            FD_HIP(hipStreamWaitEvent(e->stream, h->evs, 0));
+        }
+        FD_HIP(hipEventRecord(e->ev0, e->stream));
+        if (sequential) {
+            // the baseline: one launch per domain per iteration, one stream
+            for (int64_t i = 0; i < nl; ++i)
+                for (fd_handle* h : e->hs) {
+                    KBufs b{};
+                    b.src = h->pop[h->cur];
+                    b.dst = h->pop[1 - h->cur];
+                    b.flags = h->flags;
+                    b.side_in = h->side[h->sc];
+                    b.side_out = h->side[1 - h->sc];
+                    b.uprof = h->uprof;
+                    const int wx = h->wx, rows = h->rows;
+                    h->wx = e->wx;
+                    h->rows = e->rows;
+                    rc = launch_step(h, 0, b, 0, -1, e->stream);
+                    h->wx = wx;
+                    h->rows = rows;
+                    if (rc) return rc;
+                    h->cur ^= 1;
+                    h->sc ^= 1;
+                }
+            e->flips += nl;
+        } else {
+            int64_t i = 0;
+            if ((e->flips & 1) != 0) {
+                if ((rc = ens_launch(e, 1))) return rc;
+                ++i;
+                ++e->flips;
+            }
+            if (e->use_graph && nl - i >= 2) {
+                if ((rc = ens_graph(e))) return rc;
+                for (; nl - i >= 2; i += 2, e->flips += 2) FD_HIP(hipGraphLaunch(e->exec, e->stream));
+            }
+            for (; i < nl; ++i, ++e->flips)
+                if ((rc = ens_launch(e, (int)(e->flips & 1)))) return rc;
+            for (fd_handle* h : e->hs) {
+                h->cur ^= (int)(nl & 1);
+                h->sc ^= (int)(nl & 1);
+            }
+        }
+        FD_HIP(hipEventRecord(e->ev1, e->stream));
+        // later work on a handle's own stream waits for the ensemble
+        FD_HIP(hipEventRecord(e->evd, e->stream));
+        for (fd_handle* h : e->hs) FD_HIP(hipStreamWaitEvent(h->stream, e->evd, 0));
+        e->timed_launches = (int)nl;
+        e->have_timing = true;
+    }
+    for (fd_handle* h : e->hs) {
+        h->N += nsteps;
+        h->have_timing = false;
+    }
+    return FD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fd_ens_create(fd_handle* const* hs, int32_t n, fd_ensemble** out) {
+    if (!out) return fail(FD_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!hs || n < 1) return fail(FD_E_ARG, "need at least one handle");
+    std::vector<fd_handle*> v(hs, hs + n);
+    for (int i = 0; i < n; ++i) {
+        if (!v[i]) return fail(FD_E_ARG, "handle is NULL");
+        if (v[i]->device != v[0]->device) return fail(FD_E_ARG, "the handles of an ensemble must share a device");
+        for (int j = 0; j < i; ++j)
+            if (v[j] == v[i]) return fail(FD_E_ARG, "a handle appears twice in the ensemble");
+    }
+    fd_ensemble* e = new (std::nothrow) fd_ensemble();
+    if (!e) return fail(FD_E_ARG, "out of host memory");
+    e->hs = v;
+    e->cur0.assign(n, 0);
+    e->device = v[0]->device;
+#define FD_HIP_E(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            fd_ens_destroy(e);                                                             \
+            return fail(FD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+        }                                                                                  \
+    } while (0)
+    FD_HIP_E(hipSetDevice(e->device));
+    FD_HIP_E(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    FD_HIP_E(hipEventCreate(&e->ev0));
+    FD_HIP_E(hipEventCreate(&e->ev1));
+    FD_HIP_E(hipEventCreateWithFlags(&e->evd, hipEventDisableTiming));
+    FD_HIP_E(hipMalloc(&e->table, sizeof(EnsDomain) * (size_t)n));
+#undef FD_HIP_E
+    *out = e;
+    return FD_OK;
+}
+
+int fd_ens_step(fd_ensemble* e, int32_t nsteps) { return ens_step(e, nsteps, false); }
+
+int fd_ens_step_sequential(fd_ensemble* e, int32_t nsteps) { return ens_step(e, nsteps, true); }
+
+int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows) {
+    if (!e) return fail(FD_E_ARG, "ensemble is NULL");
+    if (waves_x != 1 && waves_x != 2 && waves_x != 4 && waves_x != 8)
+        return fail(FD_E_ARG, "waves_x must be 1, 2, 4 or 8");
+    if (rows < 1) return fail(FD_E_ARG, "rows must be >= 1");
+    e->wx = waves_x;
+    e->rows = rows;
+    e->table_ok = false;
+    return FD_OK;
+}
+
+int fd_ens_use_graph(fd_ensemble* e, int32_t on) {
+    if (!e) return fail(FD_E_ARG, "ensemble is NULL");
+    e->use_graph = on != 0;
+    return FD_OK;
+}
+
+int fd_ens_synchronize(fd_ensemble* e) {
+    if (!e) return fail(FD_E_ARG, "ensemble is NULL");
+    FD_HIP(hipSetDevice(e->device));
+    FD_HIP(hipStreamSynchronize(e->stream));
+    return FD_OK;
+}
+
+double fd_ens_last_step_ms(const fd_ensemble* e) {
+    if (!e || !e->have_timing || e->timed_launches <= 0) return -1.0;
+    if (hipEventSynchronize(e->ev1) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e->ev0, e->ev1) != hipSuccess) return -1.0;
+    return (double)ms / e->timed_launches;
+}
+
+void fd_ens_destroy(fd_ensemble* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->table) (void)hipFree(e->table);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->evd) (void)hipEventDestroy(e->evd);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
 }
 
 }  // extern "C"

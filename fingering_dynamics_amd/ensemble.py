@@ -1,0 +1,166 @@
+"""Ensembles of independent domains advanced by one launch per iteration,
+and the seeded generator of the many-domain workload.
+
+The reference runs one domain per process (fingering_periodic.py:454-479).
+Here any number of ``Compute`` solvers on one GPU - different sizes,
+viscosity ratios, wall values, obstacles and boundary conditions - are
+batched: one kernel launch walks a device table of the domains and advances
+all of them by one iteration, and step loops are replayed from a captured
+HIP graph.  ``step_sequential`` runs the same iterations as one launch per
+domain on the same stream, the baseline the batched launch is measured
+against; both give bit-identical states.
+
+``autotune`` picks the launch shape by timing a few real iterations per
+candidate; the results do not depend on the shape (every node's update is
+the same arithmetic whatever the tiling), so the trial iterations are simply
+part of the run.
+
+Not every domain of the many-domain workload stays finite: dense beds of
+small obstacles at these interface constants drive the scheme unstable in
+about 18 % of the seeded domains within 10,000 iterations (the FP64 oracle
+diverges on the same domains; with the Gaussian velocity inlet and outlet of
+fingering_periodic.py it is about half of them).  ``Compute.mass()`` counts
+the non-finite nodes of each domain; the other domains are unaffected, and
+a diverged domain costs the same time per iteration.
+"""
+import math
+
+import numpy as np
+
+from . import _native, geometry, wetting
+from .params import LbmParams, Y_WALLS, PROFILE_CONST, OUTLET_PRESSURE
+
+TILE_CANDIDATES = ((4, 16), (4, 8), (4, 32), (8, 16), (2, 16), (1, 16))
+
+
+class Ensemble:
+    """Batched stepping of several ``Compute`` solvers (same device)."""
+
+    def __init__(self, computes):
+        self._lib = _native.load()
+        self.computes = list(computes)
+        if not self.computes:
+            raise ValueError("an ensemble needs at least one solver")
+        arr = (_native.ctypes.c_void_p * len(self.computes))(*[c._h.value for c in self.computes])
+        h = _native.ctypes.c_void_p()
+        _native.check(self._lib.fd_ens_create(arr, len(self.computes), _native.ctypes.byref(h)))
+        self._h = h
+        self.tile = (4, 16)
+
+    @property
+    def nodes(self):
+        return sum(c.H * c.W for c in self.computes)
+
+    def step(self, n=1):
+        """n iterations of every domain, one launch per iteration (asynchronous)."""
+        _native.check(self._lib.fd_ens_step(self._h, int(n)))
+        self._invalidate()
+
+    def step_sequential(self, n=1):
+        """The same iterations as one launch per domain per iteration."""
+        _native.check(self._lib.fd_ens_step_sequential(self._h, int(n)))
+        self._invalidate()
+
+    def _invalidate(self):
+        for c in self.computes:
+            c._cache = None
+
+    def set_tile(self, waves_x, rows):
+        _native.check(self._lib.fd_ens_set_tile(self._h, int(waves_x), int(rows)))
+        self.tile = (int(waves_x), int(rows))
+
+    def use_graph(self, on=True):
+        _native.check(self._lib.fd_ens_use_graph(self._h, 1 if on else 0))
+
+    def synchronize(self):
+        _native.check(self._lib.fd_ens_synchronize(self._h))
+
+    def last_step_ms(self):
+        return float(self._lib.fd_ens_last_step_ms(self._h))
+
+    def autotune(self, candidates=TILE_CANDIDATES, steps=20):
+        """Time ``steps`` real iterations per launch shape and keep the
+        fastest.  Returns {tile: ms per iteration}."""
+        times = {}
+        for wx, rows in candidates:
+            self.set_tile(wx, rows)
+            self.step(2)                         # table rebuild and graph capture
+            self.step(steps)
+            times[(wx, rows)] = self.last_step_ms()
+        best = min(times, key=times.get)
+        self.set_tile(*best)
+        return times
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fd_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def domain_spec(k, seed=3):
+    """Domain k of the many-domain workload: W in {256, 288, ..., 1024} and
+    H in {128, 160, ..., 512} drawn uniformly and independently; porosity
+    U(0.6, 0.85) as the target (random placement of non-overlapping
+    obstacles jams below some coverage in narrow domains; the porosity
+    reached is what the mask holds); squares or circles with probability
+    1/2 (seed 1000*seed+k);
+    viscosity ratio M = exp(U(0, ln 20)); contact angle U(31, 90) degrees
+    through the defending fluid, turned into psi_wall by the droplet
+    calibration (60..90 degrees through the invading fluid stays inside the
+    calibrated, stable range; see bench.wall_value).  The flow is that of
+    the single-domain workload: constant Zou-He inlet u = 0.001, pressure
+    outlet, half-way walls on rows 0 and H-1, complete bounce-back links."""
+    rng = np.random.default_rng([seed, k])
+    W = int(rng.integers(256 // 32, 1024 // 32 + 1)) * 32
+    H = int(rng.integers(128 // 32, 512 // 32 + 1)) * 32
+    porosity = float(rng.uniform(0.6, 0.85))
+    circles = bool(rng.random() < 0.5)
+    M = float(math.exp(rng.uniform(0.0, math.log(20.0))))
+    theta = float(rng.uniform(31.0, 90.0))
+    return dict(k=k, H=H, W=W, porosity=porosity, circles=circles, M=M, theta=theta,
+                obstacle_seed=1000 * seed + k)
+
+
+def build_domain(spec):
+    """(mask, links, params) of one domain_spec."""
+    H, W = spec["H"], spec["W"]
+    x_lo, x_hi = 24, W - 25
+    if spec["circles"]:
+        rmax = max(4, min(12, H // 12))
+        circles = geometry.random_circles(H, W, 3, rmax, spec["porosity"], seed=spec["obstacle_seed"],
+                                          margin=1, x_lo=x_lo, x_hi=x_hi, max_tries=20000, strict=False)
+        bpa, _, _, _ = geometry.Createblock(H, W).setCirleblock(circles)
+    else:
+        bpa = _square_bed(H, W, spec["porosity"], spec["obstacle_seed"], x_lo, x_hi)
+    mask = geometry.fluid_mask(bpa)
+    links = geometry.link_bits_complete(mask)
+    params = LbmParams.periodic(H, W, M=spec["M"], psi_wall=wetting.psi_wall_for_angle(180.0 - spec["theta"]),
+                                u0=0.001, profile=PROFILE_CONST, outlet=OUTLET_PRESSURE, rho_out=1.0,
+                                y_bc=Y_WALLS, wall_lo=0, wall_hi=H - 1, n_inv_cols=16)
+    return mask, links, params
+
+
+def _square_bed(H, W, porosity, seed, x_lo, x_hi, size=6):
+    """Seeded non-overlapping size x size squares between columns x_lo and
+    x_hi, rows 2..H-3, until the solid fraction of the whole domain reaches
+    1 - porosity (or no further square fits)."""
+    rng = np.random.default_rng(seed)
+    block = np.zeros((H, W), dtype=np.uint8)
+    target = (1.0 - porosity) * H * W
+    solid = 0
+    tries = 0
+    while solid + size * size <= target and tries < 20000:
+        tries += 1
+        x0 = int(rng.integers(x_lo, x_hi - size))
+        y0 = int(rng.integers(2, H - 2 - size))
+        if block[y0 - 1:y0 + size + 1, x0 - 1:x0 + size + 1].any():
+            continue
+        block[y0:y0 + size, x0:x0 + size] = 1
+        solid += size * size
+    return block

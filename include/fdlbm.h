@@ -176,6 +176,46 @@ int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy);
 /* Release every device buffer of the handle.  NULL is ignored. */
 void fd_destroy(fd_handle* h);
 
+/* ---- Ensembles: many independent domains advanced together ------------
+ * No reference counterpart: the reference runs one domain per process
+ * (fingering_periodic.py:454-479).  An ensemble batches existing handles
+ * (any sizes, parameters and boundary conditions, one device) so that one
+ * kernel launch advances every domain by one iteration; the launches of a
+ * step loop are replayed from a captured HIP graph.  Each handle keeps its
+ * own state, and fd_get_fields / fd_phase_mass / fd_save_state ... on a
+ * handle stay valid (they are ordered after the ensemble's work).  Destroy
+ * the ensemble before its handles. */
+typedef struct fd_ensemble fd_ensemble;
+
+/* Batch n distinct, initialised handles of one device. */
+int fd_ens_create(fd_handle* const* hs, int32_t n, fd_ensemble** out);
+
+/* nsteps iterations of every domain: one launch per iteration covering all
+ * of them (graph-replayed in pairs unless disabled).  All handles must be
+ * at the same iteration count.  Asynchronous. */
+int fd_ens_step(fd_ensemble* e, int32_t nsteps);
+
+/* The same iterations as one launch per domain per iteration, in sequence
+ * on the ensemble's stream: the baseline a batched launch is measured
+ * against.  Results are identical to fd_ens_step. */
+int fd_ens_step_sequential(fd_ensemble* e, int32_t nsteps);
+
+/* Launch shape of the batched kernel, as fd_set_tile (default 4 x 16). */
+int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows);
+
+/* 1: replay step loops from a captured graph (default); 0: launch eagerly. */
+int fd_ens_use_graph(fd_ensemble* e, int32_t on);
+
+/* Wait for the ensemble's queued work. */
+int fd_ens_synchronize(fd_ensemble* e);
+
+/* Average device time per iteration (ms, all domains) of the last
+ * fd_ens_step / fd_ens_step_sequential call; < 0 if unavailable. */
+double fd_ens_last_step_ms(const fd_ensemble* e);
+
+/* Release the ensemble (not its handles).  NULL is ignored. */
+void fd_ens_destroy(fd_ensemble* e);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* fd_last_error(void);
 
