@@ -1,6 +1,7 @@
 """Build the native libraries in-tree.
 
 libfdlbm.so     HIP kernels + C ABI for gfx950 (hipcc)
+libfdsurr.so    BF16 MFMA convolutions of the phase-field surrogate (hipcc)
 libfdnode.so    host FP64 build of the per-node arithmetic (g++), loaded only
                 by the CPU tests
 
@@ -48,6 +49,15 @@ def build_hip():
     return out
 
 
+def build_surrogate():
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libfdsurr.so")
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "fdsurr.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_node_host():
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libfdnode.so")
@@ -60,7 +70,7 @@ def build_node_host():
 
 
 def build_all():
-    return [build_hip(), build_node_host()]
+    return [build_hip(), build_surrogate(), build_node_host()]
 
 
 if __name__ == "__main__":

@@ -1,0 +1,295 @@
+// fdsurr.hip - gfx950 BF16 MFMA 3x3 convolutions of the phase-field
+// surrogate (declarations and conventions: include/fdsurr.h).
+//
+// All three passes are GEMMs on v_mfma_f32_16x16x32_bf16 (FP32 accumulate).
+// Operand layout of one 16x16x32 MFMA (measured on MI355X):
+//   A[i][k] (16 x 32): lane i + 16*(k/8), element k%8
+//   B[k][j] (32 x 16): lane j + 16*(k/8), element k%8
+//   D[i][j] (16 x 16): lane j + 16*(i/4), element i%4
+// so a lane holds 8 consecutive k of one row (A) or column (B): in NHWC
+// with C % 8 == 0 those are 8 channels of one pixel and one tap - a single
+// 16-byte load.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "fdsurr.h"
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define FDS_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(FDS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+__device__ __forceinline__ v8bf bf8(u4v v) { return __builtin_bit_cast(v8bf, v); }
+
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// ---------------------------------------------------------------- forward
+// y[p][co] = act(bias[co] + sum_k A[p][k] w[co][k]), p over N*H*W (NHWC),
+// k = tap*Cin + ci with tap = 3*(dh+1) + (dw+1) and A[p][k] = x at pixel
+// (n, h+dh, w+dw), channel ci (0 outside the image).  One wave computes 16
+// consecutive pixels x 64 output channels (four 16-column MFMA tiles); a
+// workgroup of four waves covers 64 pixels.  The K loop takes 32 k per
+// step; k beyond 9*Cin and channels beyond Cout load zeros.
+__global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+                                                const float* __restrict__ bias, float* __restrict__ y32,
+                                                uint16_t* __restrict__ y16, int N, int H, int W, int Cin, int Cout,
+                                                int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    const int co0 = blockIdx.y * 64;
+    const int K = 9 * Cin;
+    const int64_t pa = p0 + (lane & 15);
+    const bool pin = pa < P;
+    int n = 0, h = 0, wc = 0;
+    if (pin) {
+        wc = (int)(pa % W);
+        const int64_t t = pa / W;
+        h = (int)(t % H);
+        n = (int)(t / H);
+    }
+    v4f acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < K; kc += 32) {
+        const int k0 = kc + 8 * (lane >> 4);
+        u4v av = {0u, 0u, 0u, 0u};
+        if (pin && k0 < K) {
+            const int tap = k0 / Cin, ci = k0 - tap * Cin;
+            const int hh = h + tap / 3 - 1, ww = wc + tap % 3 - 1;
+            if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                av = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int co = co0 + 16 * t + (lane & 15);
+            u4v bv = {0u, 0u, 0u, 0u};
+            if (co < Cout && k0 < K) bv = *reinterpret_cast<const u4v*>(w + (int64_t)co * K + k0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf8(av), bf8(bv), acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int co = co0 + 16 * t + (lane & 15);
+        if (co >= Cout) continue;
+        const float b = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t p = p0 + 4 * (lane >> 4) + r;
+            if (p >= P) continue;
+            float v = acc[t][r] + b;
+            if (relu) v = fmaxf(v, 0.f);
+            if (y32) y32[p * Cout + co] = v;
+            if (y16) y16[p * Cout + co] = to_bf16(v);
+        }
+    }
+}
+
+// wt[ci][tap][co] = w[co][8 - tap][ci]
+__global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt, int Cin,
+                                                    int Cout) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)Cout * 9 * Cin;
+    if (i >= total) return;
+    const int ci = (int)(i % Cin);
+    const int tap = (int)((i / Cin) % 9);
+    const int co = (int)(i / (9 * (int64_t)Cin));
+    wt[((int64_t)ci * 9 + (8 - tap)) * Cout + co] = w[i];
+}
+
+// ------------------------------------------------------- weight gradient
+// dw[co][tap][ci] = sum_p dy[p][co] x[p + d(tap)][ci]: a GEMM whose
+// reduction runs over pixels, which are the strided dimension of both
+// operands, so each 32-pixel step stages dy (32 x 64 channels) and the
+// shifted x (32 x 64 channels) in LDS and the fragments are read from there
+// (rows padded to 72 elements).  A workgroup owns one tap, 64 co x 64 ci and
+// one slice of pixels; wave w computes co rows 16w..16w+15.  Slices write
+// partial sums that reduce_slices adds in a fixed order.
+constexpr int kLds = 72;
+
+__global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                                                  float* __restrict__ part, int N, int H, int W, int Cin, int Cout,
+                                                  int64_t chunk) {
+    __shared__ __align__(16) uint16_t sdy[32][kLds];
+    __shared__ __align__(16) uint16_t sx[32][kLds];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ncb = (Cout + 63) / 64;
+    int gy = blockIdx.y;
+    const int cb = gy % ncb;
+    gy /= ncb;
+    const int tap = gy % 9;
+    const int ib = gy / 9;
+    const int co0 = cb * 64, ci0 = ib * 64, dh = tap / 3 - 1, dw = tap % 3 - 1;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t pbeg = (int64_t)blockIdx.x * chunk;
+    const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
+    v4f acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int pr = threadIdx.x >> 3, cg = (threadIdx.x & 7) * 8;
+    for (int64_t pc = pbeg; pc < pend; pc += 32) {
+        const int64_t p = pc + pr;
+        u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u};
+        if (p < pend) {
+            if (co0 + cg < Cout) vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+            const int wc = (int)(p % W);
+            const int64_t t = p / W;
+            const int h = (int)(t % H), n = (int)(t / H);
+            const int hh = h + dh, ww = wc + dw;
+            if (hh >= 0 && hh < H && ww >= 0 && ww < W && ci0 + cg < Cin)
+                vx = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+        }
+        *reinterpret_cast<u4v*>(&sdy[pr][cg]) = vdy;
+        *reinterpret_cast<u4v*>(&sx[pr][cg]) = vx;
+        __syncthreads();
+        v8bf a;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = __builtin_bit_cast(__bf16, sdy[8 * (lane >> 4) + e][16 * wave + (lane & 15)]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v8bf b;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) b[e] = __builtin_bit_cast(__bf16, sx[8 * (lane >> 4) + e][16 * t + (lane & 15)]);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int64_t K = 9 * (int64_t)Cin;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int ci = ci0 + 16 * t + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
+            if (co < Cout && ci < Cin) part[((int64_t)blockIdx.x * Cout + co) * K + tap * Cin + ci] = acc[t][r];
+        }
+    }
+}
+
+// per-slice sums of dy over pixels (bias gradient): bpart[s][co]
+__global__ __launch_bounds__(256) void bias_part(const uint16_t* __restrict__ dy, float* __restrict__ bpart, int64_t P,
+                                                 int Cout, int64_t chunk) {
+    const int64_t pbeg = (int64_t)blockIdx.x * chunk;
+    const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
+    for (int co = threadIdx.x; co < Cout; co += blockDim.x) {
+        float s = 0.f;
+        for (int64_t p = pbeg; p < pend; ++p) s += (float)__builtin_bit_cast(__bf16, dy[p * Cout + co]);
+        bpart[(int64_t)blockIdx.x * Cout + co] = s;
+    }
+}
+
+// out[i] = sum over s of part[s][i], s in order
+__global__ __launch_bounds__(256) void reduce_slices(const float* __restrict__ part, float* __restrict__ out,
+                                                     int64_t n, int S) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < S; ++k) s += part[(int64_t)k * n + i];
+    out[i] = s;
+}
+
+int check(int N, int H, int W, int Cin, int Cout) {
+    if (N < 1 || H < 1 || W < 1 || Cin < 8 || Cout < 1) return fail(FDS_E_ARG, "bad shape");
+    if (Cin % 8) return fail(FDS_E_ARG, "Cin must be a multiple of 8");
+    if ((int64_t)N * H * W >= ((int64_t)1 << 40)) return fail(FDS_E_ARG, "too many pixels");
+    return FDS_OK;
+}
+
+// pixel slices of the weight gradient: 32-pixel steps, at most 256 slices
+void slices(int64_t P, int64_t* chunk, int* S) {
+    int64_t c = (P + 255) / 256;
+    c = std::max<int64_t>((c + 31) / 32 * 32, 512);
+    *chunk = c;
+    *S = (int)((P + c - 1) / c);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fds_last_error(void) { return g_err.c_str(); }
+
+int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* y32, uint16_t* y16, int32_t N,
+                    int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t relu, void* stream) {
+    int rc = check(N, H, W, Cin, Cout);
+    if (rc) return rc;
+    if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
+    if (!x || !w || (!y32 && !y16)) return fail(FDS_E_ARG, "NULL pointer");
+    const int64_t P = (int64_t)N * H * W;
+    const dim3 grid((unsigned)((P + 63) / 64), (unsigned)((Cout + 63) / 64));
+    hipLaunchKernelGGL(conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, x, w, bias, y32, y16, N, H, W, Cin, Cout,
+                       relu);
+    FDS_HIP(hipGetLastError());
+    return FDS_OK;
+}
+
+int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout, void* stream) {
+    if (!w || !wt || Cin < 1 || Cout < 1) return fail(FDS_E_ARG, "bad arguments");
+    const int64_t total = (int64_t)Cout * 9 * Cin;
+    hipLaunchKernelGGL(flip_weights, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, w, wt,
+                       Cin, Cout);
+    FDS_HIP(hipGetLastError());
+    return FDS_OK;
+}
+
+int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, float* dx32, uint16_t* dx16, int32_t N, int32_t H,
+                         int32_t W, int32_t Cin, int32_t Cout, void* stream) {
+    // the transposed convolution is a forward convolution of dy (Cout
+    // channels) with the flipped weights (Cin outputs)
+    return fds_conv3x3_fwd(dy, wt, nullptr, dx32, dx16, N, H, W, Cout, Cin, 0, stream);
+}
+
+int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
+    int64_t chunk;
+    int S;
+    slices((int64_t)N * H * W, &chunk, &S);
+    return (int64_t)S * Cout * (9 * (int64_t)Cin + 1);
+}
+
+int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, float* dw, float* dbias, float* scratch, int32_t N,
+                           int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream) {
+    int rc = check(N, H, W, Cin, Cout);
+    if (rc) return rc;
+    if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
+    if (!x || !dy || !dw || !scratch) return fail(FDS_E_ARG, "NULL pointer");
+    const int64_t P = (int64_t)N * H * W;
+    int64_t chunk;
+    int S;
+    slices(P, &chunk, &S);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nw = (int64_t)Cout * 9 * Cin;
+    const dim3 grid((unsigned)S, (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
+    hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, scratch, N, H, W, Cin, Cout, chunk);
+    FDS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, scratch, dw, nw, S);
+    FDS_HIP(hipGetLastError());
+    if (dbias) {
+        float* bp = scratch + (int64_t)S * nw;
+        hipLaunchKernelGGL(bias_part, dim3((unsigned)S), dim3(256), 0, st, dy, bp, P, Cout, chunk);
+        FDS_HIP(hipGetLastError());
+        hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((Cout + 255) / 256)), dim3(256), 0, st, bp, dbias,
+                           (int64_t)Cout, S);
+        FDS_HIP(hipGetLastError());
+    }
+    return FDS_OK;
+}
+
+}  // extern "C"

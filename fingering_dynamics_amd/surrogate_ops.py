@@ -1,0 +1,110 @@
+"""ctypes binding of libfdsurr.so (include/fdsurr.h): BF16 MFMA 3x3
+convolutions of the phase-field surrogate, forward and backward.
+
+Tensors are PyTorch device tensors (used for memory and streams only):
+activations NHWC bfloat16, weights [Cout][3][3][Cin] bfloat16, gradients
+and biases float32.  Every call runs on torch's current stream.  There is
+no fallback: a missing library raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libfdsurr.so")
+_lib = None
+
+_VP, _I32 = ctypes.c_void_p, ctypes.c_int32
+EXPORTS = {
+    "fds_conv3x3_fwd": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 6 + [_VP]),
+    "fds_flip_weights": (ctypes.c_int, [_VP, _VP, _I32, _I32, _VP]),
+    "fds_conv3x3_bwd_data": (ctypes.c_int, [_VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_bwd_weight_scratch": (ctypes.c_int64, [_I32] * 5),
+    "fds_conv3x3_bwd_weight": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_last_error": (ctypes.c_char_p, []),
+}
+
+
+class FdsError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise FdsError("%s is missing: run python -m fingering_dynamics_amd.build" % _LIB_PATH)
+    lib = ctypes.CDLL(_LIB_PATH)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FdsError(load().fds_last_error().decode())
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _nhwc(x, dtype):
+    if x.dtype != dtype or not x.is_contiguous() or x.dim() != 4:
+        raise ValueError("expected a contiguous 4-d %s tensor" % dtype)
+    return x.shape
+
+
+def conv3x3(x, w, bias=None, relu=False, out_dtype=torch.bfloat16):
+    """y = act(conv3x3(x, w) + bias); x [N,H,W,Cin] bf16, w [Cout,3,3,Cin] bf16."""
+    N, H, W, Cin = _nhwc(x, torch.bfloat16)
+    Cout = w.shape[0]
+    if tuple(w.shape) != (Cout, 3, 3, Cin) or w.dtype != torch.bfloat16 or not w.is_contiguous():
+        raise ValueError("w must be contiguous bf16 [Cout, 3, 3, Cin]")
+    y = torch.empty((N, H, W, Cout), dtype=out_dtype, device=x.device)
+    y32 = y if out_dtype == torch.float32 else None
+    y16 = y if out_dtype == torch.bfloat16 else None
+    _check(load().fds_conv3x3_fwd(_p(x), _p(w), _p(bias), _p(y32), _p(y16), N, H, W, Cin, Cout,
+                                  1 if relu else 0, _stream()))
+    return y
+
+
+def flip_weights(w):
+    """[Cin,3,3,Cout] weights of the transposed convolution."""
+    Cout, _, _, Cin = w.shape
+    wt = torch.empty((Cin, 3, 3, Cout), dtype=torch.bfloat16, device=w.device)
+    _check(load().fds_flip_weights(_p(w), _p(wt), Cin, Cout, _stream()))
+    return wt
+
+
+def conv3x3_bwd_data(dy, wt, out_dtype=torch.bfloat16):
+    """dx of y = conv3x3(x, w), with wt = flip_weights(w)."""
+    N, H, W, Cout = _nhwc(dy, torch.bfloat16)
+    Cin = wt.shape[0]
+    dx = torch.empty((N, H, W, Cin), dtype=out_dtype, device=dy.device)
+    _check(load().fds_conv3x3_bwd_data(_p(dy), _p(wt), _p(dx if out_dtype == torch.float32 else None),
+                                       _p(dx if out_dtype == torch.bfloat16 else None), N, H, W, Cin, Cout,
+                                       _stream()))
+    return dx
+
+
+def conv3x3_bwd_weight(x, dy, with_bias=True, scratch=None):
+    """(dw [Cout,3,3,Cin] float32, dbias [Cout] float32 or None)."""
+    N, H, W, Cin = _nhwc(x, torch.bfloat16)
+    Cout = dy.shape[3]
+    n = int(load().fds_bwd_weight_scratch(N, H, W, Cin, Cout))
+    if scratch is None or scratch.numel() < n:
+        scratch = torch.empty(n, dtype=torch.float32, device=x.device)
+    dw = torch.empty((Cout, 3, 3, Cin), dtype=torch.float32, device=x.device)
+    db = torch.empty(Cout, dtype=torch.float32, device=x.device) if with_bias else None
+    _check(load().fds_conv3x3_bwd_weight(_p(x), _p(dy), _p(dw), _p(db), _p(scratch), N, H, W, Cin, Cout,
+                                         _stream()))
+    return dw, db

@@ -1,0 +1,62 @@
+/* fdsurr.h - C ABI of the gfx950 BF16 MFMA kernels of the convolutional
+ * surrogate (3x3 convolutions of a U-Net over the coarse phase field).
+ *
+ * No reference counterpart: the reference has no learned model.  These are
+ * the building blocks of the surrogate that fast-forwards the coarse phase
+ * field of the lattice-Boltzmann solver (libfdlbm.so).
+ *
+ * Conventions
+ *   activations  NHWC, bfloat16 (raw uint16 bit patterns), C % 8 == 0
+ *   weights      [Cout][3][3][Cin] bfloat16, Cout % 16 == 0
+ *   padding      1 (zeros), stride 1: output H x W equals the input's
+ *   accumulation FP32 (MFMA v_mfma_f32_16x16x32_bf16)
+ *   pointers     device memory owned by the caller; `stream` is a
+ *                hipStream_t (NULL: the default stream); every call is
+ *                asynchronous on it
+ *   returns      0, or a negative code with a message from fds_last_error()
+ */
+#ifndef FDSURR_H
+#define FDSURR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDS_OK 0
+#define FDS_E_ARG (-1)
+#define FDS_E_HIP (-2)
+
+/* y = act(conv3x3(x, w) + bias): y32 (float NHWC) and/or y16 (bf16 NHWC),
+ * either may be NULL; bias may be NULL; relu != 0 applies max(0, .). */
+int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* y32, uint16_t* y16,
+                    int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t relu, void* stream);
+
+/* wt[ci][tap][co] = w[co][8 - tap][ci]: the weights of the transposed
+ * convolution, so that conv3x3(dy, wt) is the gradient of the input. */
+int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout, void* stream);
+
+/* dx = conv3x3(dy, wt) with wt from fds_flip_weights (dy has Cout
+ * channels, dx has Cin).  dx32 and/or dx16 may be NULL. */
+int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, float* dx32, uint16_t* dx16, int32_t N, int32_t H,
+                         int32_t W, int32_t Cin, int32_t Cout, void* stream);
+
+/* Number of floats of scratch fds_conv3x3_bwd_weight needs. */
+int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
+
+/* dw[co][tap][ci] = sum over pixels of dy[p][co] * x[p + d(tap)][ci]
+ * (float, overwritten); dbias[co] = sum over pixels of dy[p][co] (float,
+ * overwritten, may be NULL).  scratch: fds_bwd_weight_scratch floats.
+ * Deterministic: fixed-order partial sums, no atomics. */
+int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, float* dw, float* dbias, float* scratch,
+                           int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream);
+
+/* Message of the last failed call on this thread. */
+const char* fds_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDSURR_H */

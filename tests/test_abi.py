@@ -87,3 +87,23 @@ def test_no_gpu_means_an_error_not_a_fallback(lib_path):
     p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "raised" in p.stdout
+
+
+SURR_HEADER = os.path.join(ROOT, "include", "fdsurr.h")
+SURR_LIB = os.path.join(ROOT, "fingering_dynamics_amd", "_lib", "libfdsurr.so")
+
+
+def test_surrogate_library_exports_its_header():
+    """libfdsurr.so exports every function include/fdsurr.h declares, and
+    the binding in surrogate_ops declares exactly those."""
+    if not os.path.exists(SURR_LIB):
+        from fingering_dynamics_amd import build
+        build.build_surrogate()
+    text = re.sub(r"/\*.*?\*/", "", open(SURR_HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(fds_[a-z_0-9]+)\s*\(", text)))
+    assert "fds_conv3x3_fwd" in names and "fds_conv3x3_bwd_weight" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", SURR_LIB], capture_output=True, text=True).stdout
+    assert set(names) <= set(re.findall(r"\bT (fds_[a-z_0-9]+)", out))
+    from fingering_dynamics_amd import surrogate_ops
+    assert set(surrogate_ops.EXPORTS) == set(names)
+    surrogate_ops.load()

@@ -1,0 +1,81 @@
+"""BF16 MFMA convolutions against float64 CPU convolutions of the same
+bf16-rounded operands (forward, input gradient, weight and bias gradients).
+
+Tolerance per output element: the FP32 accumulation bound
+K * 2^-23 * sum |products| (K = 9 Cin terms, or the pixel count for the
+weight gradient), from a second convolution of the absolute values; a
+transposed weight, a flipped tap or a shifted pixel misses it by orders of
+magnitude.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, 8, 12, 8, 16), (1, 5, 37, 32, 64), (2, 16, 16, 64, 128), (3, 9, 20, 16, 24)]
+
+
+def _bf(t):
+    return t.to(torch.bfloat16)
+
+
+def _ref(x, w, b, dy):
+    """float64 CPU forward and autograd backward; x NHWC, w [Cout,3,3,Cin]."""
+    xc = x.detach().double().cpu().permute(0, 3, 1, 2).requires_grad_(True)
+    wc = w.detach().double().cpu().permute(0, 3, 1, 2).requires_grad_(True)
+    bc = b.detach().double().cpu().requires_grad_(True)
+    y = torch.nn.functional.conv2d(xc, wc, bc, padding=1)
+    y.backward(dy.detach().double().cpu().permute(0, 3, 1, 2))
+    absy = torch.nn.functional.conv2d(xc.detach().abs(), wc.detach().abs(), None, padding=1)
+    return (y.detach().permute(0, 2, 3, 1), xc.grad.permute(0, 2, 3, 1), wc.grad.permute(0, 2, 3, 1), bc.grad,
+            absy.permute(0, 2, 3, 1))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_conv_forward_backward(shape):
+    from fingering_dynamics_amd import surrogate_ops as so
+    N, H, W, Cin, Cout = shape
+    g = torch.Generator().manual_seed(sum(shape))
+    x = _bf(torch.randn(N, H, W, Cin, generator=g)).cuda()
+    w = _bf(torch.randn(Cout, 3, 3, Cin, generator=g) * 0.2).cuda()
+    b = torch.randn(Cout, generator=g).float().cuda()
+    dy = _bf(torch.randn(N, H, W, Cout, generator=g)).cuda()
+    y_ref, dx_ref, dw_ref, db_ref, absy = _ref(x, w, b, dy)
+    K = 9 * Cin
+    eps = 2.0 ** -23
+
+    y = so.conv3x3(x, w, b, relu=False, out_dtype=torch.float32).double().cpu()
+    assert torch.all((y - y_ref).abs() <= K * eps * (absy + b.double().cpu().abs()) + 1e-30)
+    yr = so.conv3x3(x, w, b, relu=True, out_dtype=torch.float32).double().cpu()
+    assert torch.all((yr - y_ref.clamp(min=0)).abs() <= K * eps * (absy + b.double().cpu().abs()) + 1e-30)
+    y16 = so.conv3x3(x, w, b, relu=False, out_dtype=torch.bfloat16).double().cpu()
+    assert torch.all((y16 - y_ref).abs() <= 2.0 ** -8 * y_ref.abs() + K * eps * (absy + 1.0))
+
+    wt = so.flip_weights(w)
+    dx = so.conv3x3_bwd_data(dy, wt, out_dtype=torch.float32).double().cpu()
+    absdx = torch.nn.functional.conv_transpose2d(dy.double().cpu().abs().permute(0, 3, 1, 2),
+                                                 w.double().cpu().abs().permute(0, 3, 1, 2), padding=1)
+    absdx = absdx.permute(0, 2, 3, 1)
+    assert torch.all((dx - dx_ref).abs() <= 9 * Cout * eps * absdx + 1e-30)
+
+    dw, db = so.conv3x3_bwd_weight(x, dy)
+    P = N * H * W
+    xa = x.double().cpu().abs().permute(0, 3, 1, 2)
+    dya = dy.double().cpu().abs().permute(0, 3, 1, 2)
+    absdw = torch.nn.grad.conv2d_weight(xa, (Cout, Cin, 3, 3), dya, padding=1).permute(0, 2, 3, 1)
+    assert torch.all((dw.double().cpu() - dw_ref).abs() <= 2 * P * eps * absdw + 1e-30)
+    assert torch.all((db.double().cpu() - db_ref).abs() <= 2 * P * eps * dya.sum(dim=(0, 2, 3)) + 1e-30)
+    print(shape, "max |err| fwd %.2e dx %.2e dw %.2e db %.2e" % (
+        float((y - y_ref).abs().max()), float((dx - dx_ref).abs().max()),
+        float((dw.double().cpu() - dw_ref).abs().max()), float((db.double().cpu() - db_ref).abs().max())))
+
+
+def test_flip_weights_is_an_involution_with_transpose():
+    from fingering_dynamics_amd import surrogate_ops as so
+    g = torch.Generator().manual_seed(5)
+    w = _bf(torch.randn(16, 3, 3, 8, generator=g)).cuda()
+    wt = so.flip_weights(w)
+    ref = w.cpu().flip(1, 2).permute(3, 1, 2, 0).contiguous()
+    assert torch.equal(wt.cpu(), ref)
+    assert torch.equal(so.flip_weights(wt).cpu(), w.cpu())
