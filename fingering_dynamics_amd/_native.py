@@ -32,6 +32,7 @@ EXPORTS = {
     "fd_create": (ctypes.c_int, [ctypes.POINTER(FdParams), ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_void_p)]),
     "fd_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fd_init_fields": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "fd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_steps_done": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_solid_inert": (ctypes.c_int, [ctypes.c_void_p]),
@@ -42,6 +43,12 @@ EXPORTS = {
     "fd_state_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "fd_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "fd_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "fd_fields_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "fd_init_fields_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "fd_state_device_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
+    "fd_save_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "fd_load_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "fd_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_last_step_ms": (ctypes.c_double, [ctypes.c_void_p]),
     "fd_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),

@@ -583,9 +583,14 @@ __global__ __launch_bounds__(64 * WPB) void ens_kernel(const EnsDomain* __restri
 //            fields + partial sums.
 //   PRE = 0: post-collision populations of iteration 0 into b.dst and the
 //            side buffer P of the initial psi into b.side_out.
+//   ux0/uy0 (PRE = 0 only, may be NULL): restart from given fields - the
+//            velocity is taken as given and mu from psi, so a running
+//            flow continues (fd_init_fields).
 template <int PRE>
 __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const float* __restrict__ psi0,
-                                                   const float* __restrict__ rho0, int walled_init) {
+                                                   const float* __restrict__ rho0, int walled_init,
+                                                   const float* __restrict__ ux0 = nullptr,
+                                                   const float* __restrict__ uy0 = nullptr) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
     double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
@@ -622,6 +627,14 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             m.ux = 0.f;
             m.uy = walled_init ? (0.5f * m.fy) / m.rho : 0.f;
             m.p = m.rho * (1.f / 3.f);  // evaluated before mu (both inits)
+            if (!PRE && ux0) {
+                m.mu = fdn::chem_pot(psi, lap, k.c);
+                m.fx = m.mu * dpx + k.c.gx;
+                m.fy = m.mu * dpy + k.c.gy;
+                m.ux = ux0[o];
+                m.uy = uy0[o];
+                m.p = m.rho * (1.f / 3.f) + psi * m.mu;
+            }
             m.itau = fdn::inv_tau_mix(m.rho, psi, k.c);
             fdn::equilibria(m, k.c, f, g);
             if (!PRE) fdn::collide(f, g, m, k.c);
@@ -1044,8 +1057,8 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     return FD_OK;
 }
 
-int fd_init(fd_handle* h, const double* psi0, const double* rho0) {
-    if (!h || !psi0) return fail(FD_E_ARG, "handle/psi0 is NULL");
+namespace {
+int init_impl(fd_handle* h, const double* psi0, const double* rho0, const double* ux0, const double* uy0) {
     FD_HIP(hipSetDevice(h->device));
     const size_t H = h->p.H, W = h->p.W, Wp = h->Wp;
     std::vector<float> ps(h->Q, 0.f), rh(h->Q, 0.f);
@@ -1062,8 +1075,25 @@ int fd_init(fd_handle* h, const double* psi0, const double* rho0) {
     b.side_out = h->side[0];
     b.uprof = h->uprof;
     const dim3 grid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
+    const float* dux = nullptr;
+    const float* duy = nullptr;
+    std::vector<float> vx, vy;
+    if (ux0) {
+        // the field buffer's velocity planes stage the given velocity
+        vx.assign(h->Q, 0.f);
+        vy.assign(h->Q, 0.f);
+        for (size_t y = 0; y < H; ++y)
+            for (size_t x = 0; x < W; ++x) {
+                vx[y * Wp + x] = (float)ux0[y * W + x];
+                vy[y * Wp + x] = (float)uy0[y * W + x];
+            }
+        FD_HIP(hipMemcpyAsync(h->fields + 2 * h->Q, vx.data(), sizeof(float) * h->Q, hipMemcpyHostToDevice, h->stream));
+        FD_HIP(hipMemcpyAsync(h->fields + 3 * h->Q, vy.data(), sizeof(float) * h->Q, hipMemcpyHostToDevice, h->stream));
+        dux = h->fields + 2 * h->Q;
+        duy = h->fields + 3 * h->Q;
+    }
     hipLaunchKernelGGL((init_kernel<0>), grid, dim3(64, 4), 0, h->stream, h->k, b, h->psi0, h->rho0,
-                       h->p.init_mode == FD_INIT_WALLED ? 1 : 0);
+                       h->p.init_mode == FD_INIT_WALLED ? 1 : 0, dux, duy);
     FD_HIP(hipGetLastError());
     FD_HIP(hipStreamSynchronize(h->stream));  // host staging vectors go out of scope
     h->cur = 0;
@@ -1072,6 +1102,17 @@ int fd_init(fd_handle* h, const double* psi0, const double* rho0) {
     h->inited = true;
     h->have_timing = false;
     return FD_OK;
+}
+}  // namespace
+
+int fd_init(fd_handle* h, const double* psi0, const double* rho0) {
+    if (!h || !psi0) return fail(FD_E_ARG, "handle/psi0 is NULL");
+    return init_impl(h, psi0, rho0, nullptr, nullptr);
+}
+
+int fd_init_fields(fd_handle* h, const double* psi, const double* rho, const double* ux, const double* uy) {
+    if (!h || !psi || !rho || !ux || !uy) return fail(FD_E_ARG, "handle or a field is NULL");
+    return init_impl(h, psi, rho, ux, uy);
 }
 
 int fd_step(fd_handle* h, int32_t nsteps) {
@@ -1280,6 +1321,85 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
     }
     FD_HIP(hipStreamSynchronize(h->stream));
     h->N = hdr[3];
+    h->inited = true;
+    h->have_timing = false;
+    return FD_OK;
+}
+
+// ----------------------------------------------- device-array coupling
+// Everything below is enqueued on the handle's stream and returns without
+// waiting; the caller orders its own work on that stream (fd_stream).
+
+void* fd_stream(const fd_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+int fd_fields_device(fd_handle* h, float* psi, float* rho, float* ux, float* uy) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    FD_HIP(hipSetDevice(h->device));
+    int rc = compute_fields(h, nullptr);
+    if (rc) return rc;
+    float* outs[4] = {psi, rho, ux, uy};
+    for (int i = 0; i < 4; ++i)
+        if (outs[i])
+            FD_HIP(hipMemcpy2DAsync(outs[i], sizeof(float) * h->p.W, h->fields + i * h->Q, sizeof(float) * h->Wp,
+                                    sizeof(float) * h->p.W, h->p.H, hipMemcpyDeviceToDevice, h->stream));
+    return FD_OK;
+}
+
+int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, const float* ux, const float* uy) {
+    if (!h || !psi || !rho || !ux || !uy) return fail(FD_E_ARG, "handle or a field is NULL");
+    if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
+    FD_HIP(hipSetDevice(h->device));
+    // Padding columns x >= W of the staging planes are never read by the
+    // initialisation (it covers x < W only).
+    const float* src[4] = {psi, rho, ux, uy};
+    float* dst[4] = {h->psi0, h->rho0, h->fields + 2 * h->Q, h->fields + 3 * h->Q};
+    for (int i = 0; i < 4; ++i)
+        FD_HIP(hipMemcpy2DAsync(dst[i], sizeof(float) * h->Wp, src[i], sizeof(float) * h->p.W,
+                                sizeof(float) * h->p.W, h->p.H, hipMemcpyDeviceToDevice, h->stream));
+    KBufs b{};
+    b.flags = h->flags;
+    b.dst = h->pop[0];
+    b.side_out = h->side[0];
+    b.uprof = h->uprof;
+    const dim3 grid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
+    hipLaunchKernelGGL((init_kernel<0>), grid, dim3(64, 4), 0, h->stream, h->k, b, h->psi0, h->rho0,
+                       h->p.init_mode == FD_INIT_WALLED ? 1 : 0, dst[2], dst[3]);
+    FD_HIP(hipGetLastError());
+    h->cur = 0;
+    h->sc = 0;
+    h->N = 0;
+    h->have_timing = false;
+    return FD_OK;
+}
+
+// Device state blob: side[4H] then the 18 padded population planes [H][Wp].
+int64_t fd_state_device_bytes(const fd_handle* h) {
+    if (!h) return -1;
+    return (int64_t)sizeof(float) * (4 * (int64_t)h->p.H + 18 * (int64_t)h->Q);
+}
+
+int fd_save_state_device(fd_handle* h, void* dbuf, int64_t nbytes) {
+    if (!h || !dbuf) return fail(FD_E_ARG, "handle/buf is NULL");
+    if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
+    if (nbytes < fd_state_device_bytes(h)) return fail(FD_E_ARG, "buffer too small");
+    FD_HIP(hipSetDevice(h->device));
+    char* p = (char*)dbuf;
+    FD_HIP(hipMemcpyAsync(p, h->side[h->sc], sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToDevice, h->stream));
+    FD_HIP(hipMemcpyAsync(p + sizeof(float) * 4 * h->p.H, h->pop[h->cur], sizeof(float) * 18 * h->Q,
+                          hipMemcpyDeviceToDevice, h->stream));
+    return FD_OK;
+}
+
+int fd_load_state_device(fd_handle* h, const void* dbuf, int64_t nbytes, int64_t steps_done) {
+    if (!h || !dbuf) return fail(FD_E_ARG, "handle/buf is NULL");
+    if (nbytes < fd_state_device_bytes(h)) return fail(FD_E_ARG, "buffer too small");
+    if (steps_done < 1) return fail(FD_E_ARG, "a device state is restored at iteration >= 1");
+    FD_HIP(hipSetDevice(h->device));
+    const char* p = (const char*)dbuf;
+    FD_HIP(hipMemcpyAsync(h->side[h->sc], p, sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToDevice, h->stream));
+    FD_HIP(hipMemcpyAsync(h->pop[h->cur], p + sizeof(float) * 4 * h->p.H, sizeof(float) * 18 * h->Q,
+                          hipMemcpyDeviceToDevice, h->stream));
+    h->N = steps_done;
     h->inited = true;
     h->have_timing = false;
     return FD_OK;

@@ -68,6 +68,7 @@ class Compute:
         _native.check(self._lib.fd_create(_native.ctypes.byref(self._fdp), _native.ptr(solid),
                                           _native.ptr(bits), _native.ctypes.byref(h)))
         self._h = h
+        self._tstream = None
         if psi0 is None:
             psi0 = initial_psi(params, self.mask)
         if rho0 is None:
@@ -79,6 +80,16 @@ class Compute:
         psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
         rho0 = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
         _native.check(self._lib.fd_init(self._h, _native.ptr(psi0), _native.ptr(rho0)))
+        self._cache = None
+
+    def init_fields(self, psi, rho, ux, uy):
+        """Restart from fields [H][W] (fd_init_fields): the flow keeps its
+        density and velocity, the phase field is replaced; t restarts at 0."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi, rho, ux, uy)]
+        for a in arrs:
+            if a.shape != (self.H, self.W):
+                raise ValueError("fields must be [H][W]")
+        _native.check(self._lib.fd_init_fields(self._h, *[_native.ptr(a) for a in arrs]))
         self._cache = None
 
     def step(self, n=1):
@@ -170,6 +181,58 @@ class Compute:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         _native.check(self._lib.fd_load_state(self._h, _native.ptr(buf), buf.size))
         self._cache = None
+
+    # -------------------------------------------------- device arrays
+    def torch_stream(self):
+        """The handle's stream (fd_stream) as a torch stream.  Torch work on
+        the device arrays below goes inside
+        ``with torch.cuda.stream(cm.torch_stream()):`` to stay ordered with
+        the solver's launches (and its allocations with their use)."""
+        import torch
+        if self._tstream is None:
+            self._tstream = torch.cuda.ExternalStream(self._lib.fd_stream(self._h))
+        return self._tstream
+
+    def fields_device(self, out=None):
+        """(psi, rho, ux, uy) of the last completed iteration as a float32
+        [4][H][W] device tensor, enqueued on the handle's stream
+        (fd_fields_device).  `out` is reused when given."""
+        import torch
+        if out is None:
+            with torch.cuda.stream(self.torch_stream()):
+                out = torch.empty((4, self.H, self.W), dtype=torch.float32, device="cuda")
+        self._check_dev(out[0])
+        _native.check(self._lib.fd_fields_device(self._h, *[out[i].data_ptr() for i in range(4)]))
+        return out
+
+    def init_fields_device(self, psi, rho, ux, uy):
+        """init_fields from float32 [H][W] device tensors (fd_init_fields_device)."""
+        for a in (psi, rho, ux, uy):
+            self._check_dev(a)
+        _native.check(self._lib.fd_init_fields_device(self._h, psi.data_ptr(), rho.data_ptr(), ux.data_ptr(),
+                                                      uy.data_ptr()))
+        self._cache = None
+
+    def save_state_device(self, buf=None):
+        """(device blob, iteration) of the current state (fd_save_state_device)."""
+        import torch
+        n = int(self._lib.fd_state_device_bytes(self._h))
+        if buf is None or buf.numel() < n:
+            with torch.cuda.stream(self.torch_stream()):
+                buf = torch.empty(n, dtype=torch.uint8, device="cuda")
+        _native.check(self._lib.fd_save_state_device(self._h, buf.data_ptr(), n))
+        return buf, self.t
+
+    def load_state_device(self, state):
+        buf, t = state
+        _native.check(self._lib.fd_load_state_device(self._h, buf.data_ptr(), buf.numel(), int(t)))
+        self._cache = None
+
+    def _check_dev(self, a):
+        import torch
+        if (not isinstance(a, torch.Tensor) or a.dtype != torch.float32 or not a.is_cuda
+                or tuple(a.shape) != (self.H, self.W) or not a.is_contiguous()):
+            raise ValueError("expected a contiguous float32 [H][W] device tensor")
 
     def close(self):
         if getattr(self, "_h", None):

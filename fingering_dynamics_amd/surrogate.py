@@ -1,0 +1,466 @@
+"""The learned surrogate: a convolutional U-Net that fast-forwards the
+coarse phase field by K iterations, its training on trajectories of the
+lattice-Boltzmann solver, and the surrogate-accelerated run.
+
+No reference counterpart (the reference has no learned model).
+
+Network.  Input channels at the coarse resolution: psi, ux / U, uy / U,
+solid fraction, ln M / ln 20, theta / 180 deg (6, padded to 8); output: the
+change of psi over K iterations, added to the input psi.  Four levels of
+two 3x3 convolutions each (32, 64, 128, 128 channels), 2x2 average pooling
+down, nearest-neighbour upsampling and skip concatenation up, ReLU after
+every convolution but the head: about 1.2M parameters.  Every convolution
+(forward, input gradient, weight and bias gradients) runs on the BF16 MFMA
+kernels of libfdsurr.so with FP32 accumulation; activations are bf16 NHWC,
+master weights FP32 with AdamW.  Pooling, upsampling, concatenation, the
+ReLU mask and the optimiser are PyTorch tensor operations (plumbing around
+the convolutions).
+
+Data.  Domains of the many-domain workload at a fixed size (H x W, default
+256 x 512), advanced together by one batched launch per iteration; every
+`snap` iterations each domain's fields are area-averaged over f x f blocks
+(fluid nodes only) to the coarse grid.  Pairs (t, t + K).  Domains that go
+non-finite are dropped.
+
+Accelerated run.  From the LBM state at t: the surrogate predicts the coarse
+psi at t + K, which is upsampled and handed back to the solver with its
+current density and velocity (fd_init_fields); the LBM then relaxes R
+iterations, and the interface error - the fraction of coarse fluid cells
+whose phase (sign of psi) differs between the prediction and the relaxed
+field - is judged.  Above the threshold the jump is discarded: the state at
+t is restored and the plain LBM runs K + R iterations instead.  Breakthrough
+is checked after every cycle at the probe column (output.breakthrough).
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import output, surrogate_ops as so
+
+U_REF = 1e-3          # velocity scale of the inputs (the workload's inlet velocity)
+CHANNELS = (32, 64, 128, 128)
+IN_CH = 8
+
+
+# --------------------------------------------------------------- network
+class _Conv(torch.autograd.Function):
+    """3x3 convolution on the MFMA kernels; W is the FP32 master weight."""
+
+    @staticmethod
+    def forward(ctx, x, W, b, relu, out_f32):
+        w16 = W.to(torch.bfloat16).contiguous()
+        y = so.conv3x3(x, w16, b, relu=relu, out_dtype=torch.float32 if out_f32 else torch.bfloat16)
+        ctx.save_for_backward(x, w16, y if relu else None)
+        ctx.relu = relu
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        x, w16, y = ctx.saved_tensors
+        dy = dy.to(torch.bfloat16).contiguous()
+        if ctx.relu:
+            dy = (dy * (y > 0)).contiguous()
+        dx = None
+        if ctx.needs_input_grad[0]:
+            dx = so.conv3x3_bwd_data(dy, so.flip_weights(w16))
+        dw, db = so.conv3x3_bwd_weight(x, dy)
+        return dx, dw, db, None, None
+
+
+def conv(x, W, b, relu=True, out_f32=False):
+    return _Conv.apply(x, W, b, relu, out_f32)
+
+
+def _pool(x):
+    N, H, W, C = x.shape
+    return x.view(N, H // 2, 2, W // 2, 2, C).mean(dim=(2, 4))
+
+
+def _up(x):
+    return x.repeat_interleave(2, dim=1).repeat_interleave(2, dim=2)
+
+
+class UNet(torch.nn.Module):
+    def __init__(self, channels=CHANNELS, seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        c0, c1, c2, c3 = channels
+        plan = [("e0a", IN_CH, c0), ("e0b", c0, c0), ("e1a", c0, c1), ("e1b", c1, c1), ("e2a", c1, c2),
+                ("e2b", c2, c2), ("e3a", c2, c3), ("e3b", c3, c3), ("d2a", c3 + c2, c2), ("d2b", c2, c2),
+                ("d1a", c2 + c1, c1), ("d1b", c1, c1), ("d0a", c1 + c0, c0), ("d0b", c0, c0), ("head", c0, 8)]
+        self.names = [p[0] for p in plan]
+        for name, cin, cout in plan:
+            std = math.sqrt(2.0 / (9 * cin)) * (0.1 if name == "head" else 1.0)
+            self.register_parameter("w_" + name, torch.nn.Parameter(torch.randn(cout, 3, 3, cin, generator=g) * std))
+            self.register_parameter("b_" + name, torch.nn.Parameter(torch.zeros(cout)))
+
+    def _c(self, name, x, relu=True, out_f32=False):
+        return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32)
+
+    def forward(self, x):
+        """x [N, H, W, 8] bf16 (H, W multiples of 8) -> d psi [N, H, W] fp32."""
+        s0 = self._c("e0b", self._c("e0a", x))
+        s1 = self._c("e1b", self._c("e1a", _pool(s0)))
+        s2 = self._c("e2b", self._c("e2a", _pool(s1)))
+        b = self._c("e3b", self._c("e3a", _pool(s2)))
+        d = self._c("d2b", self._c("d2a", torch.cat([_up(b), s2], dim=3).contiguous()))
+        d = self._c("d1b", self._c("d1a", torch.cat([_up(d), s1], dim=3).contiguous()))
+        d = self._c("d0b", self._c("d0a", torch.cat([_up(d), s0], dim=3).contiguous()))
+        return self._c("head", d, relu=False, out_f32=True)[..., 0]
+
+    def n_params(self):
+        return sum(p.numel() for p in self.parameters())
+
+    def flops_per_sample(self, H, W):
+        """Forward FLOPs of the convolutions for one H x W sample."""
+        total, h, w = 0, H, W
+        sizes = {"e0": (H, W), "e1": (H // 2, W // 2), "e2": (H // 4, W // 4), "e3": (H // 8, W // 8),
+                 "d2": (H // 4, W // 4), "d1": (H // 2, W // 2), "d0": (H, W), "he": (H, W)}
+        for name in self.names:
+            Wt = getattr(self, "w_" + name)
+            hh, ww = sizes[name[:2]]
+            total += 2 * 9 * Wt.shape[3] * Wt.shape[0] * hh * ww
+        return total
+
+
+def inputs(psi, ux, uy, solid, M, theta_deg):
+    """Network input [N, H, W, 8] bf16 from coarse fields [N, H, W] (torch)."""
+    N, H, W = psi.shape
+    x = torch.zeros((N, H, W, IN_CH), dtype=torch.float32, device=psi.device)
+    x[..., 0] = psi
+    x[..., 1] = ux / U_REF
+    x[..., 2] = uy / U_REF
+    x[..., 3] = solid
+    x[..., 4] = (torch.log(torch.as_tensor(M, dtype=torch.float32, device=psi.device)) / math.log(20.0)).view(-1, 1, 1)
+    x[..., 5] = (torch.as_tensor(theta_deg, dtype=torch.float32, device=psi.device) / 180.0).view(-1, 1, 1)
+    return x.to(torch.bfloat16).contiguous()
+
+
+# ------------------------------------------------------ coarse <-> fine
+def coarsen(field, fluid, f):
+    """Area average of field [..., H, W] over f x f blocks, over fluid nodes
+    only (0 where a block has none) -> [..., H/f, W/f] (torch, any device).
+    fluid is a {0, 1} float tensor [H, W]."""
+    H, W = field.shape[-2:]
+    n = fluid.view(H // f, f, W // f, f).sum(dim=(1, 3))
+    s = (field * fluid).reshape(*field.shape[:-2], H // f, f, W // f, f).sum(dim=(-3, -1))
+    return torch.where(n > 0, s / n.clamp(min=1.0), torch.zeros_like(s))
+
+
+def solid_fraction(fluid, f):
+    H, W = fluid.shape
+    return 1.0 - fluid.view(H // f, f, W // f, f).sum(dim=(1, 3)) / (f * f)
+
+
+def refine(coarse, f):
+    """Nearest-neighbour upsampling of [..., h, w] by f."""
+    return coarse.repeat_interleave(f, dim=-2).repeat_interleave(f, dim=-1)
+
+
+def interface_error(psi_a, psi_b, fluid_c):
+    """Fraction of coarse fluid cells (fluid_c > 0.5) whose phase (sign of
+    psi) differs between psi_a and psi_b; a 0-d tensor."""
+    m = fluid_c > 0.5
+    return (((psi_a > 0) != (psi_b > 0)) & m).sum() / m.sum().clamp(min=1)
+
+
+# ------------------------------------------------------------------ flows
+RADII = (8, 24)       # obstacle radii of the packed beds (circles)
+U_IN = 1e-3           # constant Zou-He inlet velocity
+INV_COLS = 40         # columns of invading fluid at t = 0
+
+
+def flow_spec(k, seed=4, H=256, W=512):
+    """Flow k of the training set: a bed of circles with radius U{8..24}
+    at porosity U(0.6, 0.8), viscosity ratio M = exp(U(0, ln 20)), contact
+    angle U(30, 90) degrees through the defending fluid; obstacle seed
+    1000 * seed + k."""
+    rng = np.random.default_rng([seed, k])
+    return dict(k=k, H=H, W=W, porosity=float(rng.uniform(0.6, 0.8)),
+                M=float(math.exp(rng.uniform(0.0, math.log(20.0)))), theta=float(rng.uniform(30.0, 90.0)),
+                obstacle_seed=1000 * seed + k)
+
+
+def breakthrough_spec():
+    """The surrogate-accelerated workload: 2048 x 1024, porosity 0.7,
+    M = 15, contact angle 45 degrees, obstacle seed 5."""
+    return dict(k=0, H=1024, W=2048, porosity=0.7, M=15.0, theta=45.0, obstacle_seed=5)
+
+
+def build_flow(spec):
+    """(mask, links, params) of a flow spec: periodic in y as
+    fingering_periodic.py, constant Zou-He inlet velocity U_IN, pressure
+    outlet rho = 1, psi = +1 in the first INV_COLS columns, circles kept
+    max(64, INV_COLS + 16) columns clear of the inlet and outlet, 2 cells
+    apart and 2 cells inside the top and bottom rows, complete bounce-back
+    links.
+
+    Periodic y rather than walls: with half-way walls on rows 0 and H - 1
+    the wetting wall rows meet the psi = +1 Zou-He column at the two inlet
+    corners, where the pinned contact line drives jets of |u| ~ 0.1 (100 u0)
+    and the inlet then delivers ~4x its nominal flux; most such beds go
+    non-finite within a few thousand iterations (the FP64 oracle does the
+    same).  Periodic y has no corners: the measured inflow of psi is the
+    nominal 2 u0 H per iteration and |u| stays ~5 u0."""
+    from . import geometry, wetting
+    from .params import LbmParams, Y_PERIODIC, PROFILE_CONST, OUTLET_PRESSURE
+    H, W = spec["H"], spec["W"]
+    clear = max(64, INV_COLS + 16)
+    circles = geometry.random_circles(H, W, RADII[0], RADII[1], spec["porosity"], seed=spec["obstacle_seed"],
+                                      margin=2, x_lo=clear, x_hi=W - 1 - clear, max_tries=200000, strict=False)
+    bpa, _, _, _ = geometry.Createblock(H, W).setCirleblock(circles)
+    mask = geometry.fluid_mask(bpa)
+    links = geometry.link_bits_complete(mask)
+    params = LbmParams.periodic(H, W, M=spec["M"], psi_wall=wetting.psi_wall_for_angle(180.0 - spec["theta"]),
+                                u0=U_IN, profile=PROFILE_CONST, outlet=OUTLET_PRESSURE, rho_out=1.0,
+                                y_bc=Y_PERIODIC, n_inv_cols=INV_COLS)
+    return mask, links, params
+
+
+def probe_column(W):
+    return W - 1 - output.OUTLET_MARGIN
+
+
+def reached(fields, fluid, column, threshold=0.0):
+    """0-d bool tensor: a fluid node of the probe column has psi > threshold
+    (output.breakthrough on device arrays)."""
+    return ((fields[0][:, column] > threshold) & (fluid[:, column] > 0)).any()
+
+
+# ------------------------------------------------------------------ data
+def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4, log=print):
+    """Coarse trajectories of n_domains flows (flow_spec), advanced together
+    by one batched launch per iteration; every `snap` iterations each
+    domain's psi, ux, uy are area-averaged over f x f blocks on the GPU (on
+    the domain's own stream).  Returns a dict of device tensors: psi, ux,
+    uy [D][T][H/f][W/f], solid [D][H/f][W/f], M, theta [D], plus the
+    iteration of each snapshot.  Domains that go non-finite are dropped."""
+    from .ensemble import Ensemble
+    from .lbm import Compute
+    specs, cs = [], []
+    for k in range(n_domains):
+        spec = flow_spec(k, seed=seed, H=H, W=W)
+        m, l, p = build_flow(spec)
+        specs.append(spec)
+        cs.append(Compute(m, l, p))
+    ens = Ensemble(cs)
+    ens.set_tile(1, 16)
+    Hc, Wc = H // f, W // f
+    steps = list(range(0, max_steps + 1, snap))
+    X = torch.zeros((n_domains, len(steps), 3, Hc, Wc), dtype=torch.float32, device="cuda")
+    fluids, bufs = [], []
+    for c in cs:
+        with torch.cuda.stream(c.torch_stream()):
+            fluids.append(torch.as_tensor(c.mask, dtype=torch.float32).cuda())
+            bufs.append(None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, t in enumerate(steps):
+        if t:
+            ens.step(snap)
+        for i, c in enumerate(cs):
+            with torch.cuda.stream(c.torch_stream()):
+                bufs[i] = c.fields_device(bufs[i])
+                X[i, j] = coarsen(bufs[i][[0, 2, 3]], fluids[i], f)
+        if j % 40 == 0:
+            log("data: t=%d (%.1f s)" % (t, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    ok = [i for i, c in enumerate(cs) if c.mass()[2] == 0 and bool(torch.isfinite(X[i]).all())]
+    X = X[ok]
+    data = {"psi": X[:, :, 0].contiguous(), "ux": X[:, :, 1].contiguous(), "uy": X[:, :, 2].contiguous(),
+            "solid": torch.stack([solid_fraction(fluids[i], f) for i in ok]),
+            "M": torch.tensor([specs[i]["M"] for i in ok], dtype=torch.float32, device="cuda"),
+            "theta": torch.tensor([specs[i]["theta"] for i in ok], dtype=torch.float32, device="cuda"),
+            "steps": steps, "specs": [specs[i] for i in ok], "dropped": n_domains - len(ok), "f": f,
+            "seconds": gen_s, "lattice_updates": n_domains * H * W * max_steps}
+    ens.close()
+    for c in cs:
+        c.close()
+    return data
+
+
+def pairs(data, idx, jump, snap):
+    """All (domain, t) index pairs of domains idx with t + jump in range."""
+    d = jump // snap
+    T = data["psi"].shape[1]
+    return [(i, t) for i in idx for t in range(T - d)], d
+
+
+def batch(data, items, d, flip=None):
+    """Network input, psi(t), psi(t + K) and fluid weight of (domain, t)
+    items; flip (bool tensor [B]) mirrors samples about the channel's
+    mid-plane (y -> H - 1 - y, uy -> -uy), a symmetry of the walled flow."""
+    i = torch.tensor([a for a, _ in items], device="cuda")
+    t = torch.tensor([b for _, b in items], device="cuda")
+    p0, p1 = data["psi"][i, t], data["psi"][i, t + d]
+    ux, uy, sol = data["ux"][i, t], data["uy"][i, t], data["solid"][i]
+    if flip is not None:
+        fl = flip.view(-1, 1, 1)
+        p0, p1, ux, sol = [torch.where(fl, v.flip(-2), v) for v in (p0, p1, ux, sol)]
+        uy = torch.where(fl, -uy.flip(-2), uy)
+    x = inputs(p0, ux, uy, sol, data["M"][i], 180.0 - data["theta"][i])
+    return x, p0, p1, 1.0 - sol
+
+
+def loss_fn(model, x, psi0, psi1, wfluid):
+    pred = psi0 + model(x)
+    return ((pred - psi1) ** 2 * wfluid).sum() / wfluid.sum().clamp(min=1.0)
+
+
+def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-4, seed=0, log=print):
+    """AdamW on the FP32 master weights with a cosine schedule; random
+    (domain, t) pairs, half of them mirrored.  Returns (loss history, s)."""
+    items, d = pairs(data, train_idx, jump, snap)
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=wd)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, iters)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(iters):
+        sel = [items[j] for j in rng.integers(0, len(items), bs)]
+        flip = torch.rand(bs, device="cuda", generator=gen) < 0.5
+        x, p0, p1, wf = batch(data, sel, d, flip)
+        loss = loss_fn(model, x, p0, p1, wf)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        sched.step()
+        if it % 100 == 0 or it == iters - 1:
+            hist.append((it, float(loss)))
+            log("train it %d loss %.3e (%.1f s)" % (it, float(loss), time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    return hist, time.perf_counter() - t0
+
+
+@torch.no_grad()
+def evaluate(model, data, idx, jump, snap, bs=64):
+    """Mean fluid-weighted MSE of the surrogate and of persistence (psi(t+K)
+    = psi(t), the method without the network), and the interface error of
+    both, over all pairs of domains idx."""
+    items, d = pairs(data, idx, jump, snap)
+    acc = {"mse": 0.0, "mse_persist": 0.0, "ierr": 0.0, "ierr_persist": 0.0}
+    n = 0
+    for k in range(0, len(items), bs):
+        sel = items[k:k + bs]
+        x, p0, p1, wf = batch(data, sel, d)
+        pred = (p0 + model(x)).clamp(-1, 1)
+        w = wf.sum().clamp(min=1.0)
+        acc["mse"] += float(((pred - p1) ** 2 * wf).sum() / w) * len(sel)
+        acc["mse_persist"] += float(((p0 - p1) ** 2 * wf).sum() / w) * len(sel)
+        acc["ierr"] += float(interface_error(pred, p1, wf)) * len(sel)
+        acc["ierr_persist"] += float(interface_error(p0, p1, wf)) * len(sel)
+        n += len(sel)
+    return {k: v / max(n, 1) for k, v in acc.items()}
+
+
+# ------------------------------------------------------- plain and accelerated runs
+@torch.no_grad()
+def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
+    """Plain LBM from cm's current (fresh) state until breakthrough at the
+    probe column, checked every `chunk` iterations on device arrays.  With
+    record_every, the coarse psi of every multiple of it (offset by chunk)
+    is kept for comparisons.  Returns (breakthrough iteration or None, wall
+    seconds, {iteration: coarse psi})."""
+    col = probe_column(cm.W)
+    rec = {}
+    with torch.cuda.stream(cm.torch_stream()):
+        fluid = torch.as_tensor(cm.mask, dtype=torch.float32).cuda()
+        buf = None
+        torch.cuda.current_stream().synchronize()
+        t0 = time.perf_counter()
+        t_b = None
+        while cm.t < max_steps:
+            cm.step(chunk)
+            buf = cm.fields_device(buf)
+            if record_every and (cm.t - chunk) % record_every == 0:
+                rec[cm.t] = coarsen(buf[0], fluid, f)
+            if bool(reached(buf, fluid, col)):
+                t_b = cm.t
+                break
+            if cm.t % 100000 == 0:
+                log("plain: t=%d (%.1f s)" % (cm.t, time.perf_counter() - t0))
+        wall = time.perf_counter() - t0
+    return t_b, wall, rec
+
+
+@torch.no_grad()
+def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=0.02, max_steps=10 ** 7,
+                    mass_fix=True, log=print):
+    """Surrogate-accelerated run of solver cm (freshly initialised) until
+    breakthrough, every array on the device and every launch on the
+    solver's stream.
+
+    A cycle: the coarse fields (psi, u) go through the network, which
+    predicts the coarse psi K = jump iterations later; the predicted change
+    is added to the fine psi (nearest-neighbour upsampling, clamped to
+    [-1, 1]), optionally rescaled so that the fluid sum of psi grows by the
+    inflow's 2 u0 n_in K (invading fluid in, defending fluid out); the
+    solver restarts from (new psi, current rho, u) (fd_init_fields_device)
+    and relaxes `relax` iterations; the interface error between the
+    prediction and the relaxed coarse psi is judged, and above `threshold`
+    the state before the jump is restored and the plain LBM runs K + relax
+    iterations instead.  Breakthrough is checked after every cycle.
+    Returns (breakthrough model time or None, wall seconds, stats,
+    {model time: coarse psi})."""
+    H, W = cm.H, cm.W
+    col = probe_column(W)
+    stats = {"jumps": 0, "fallbacks": 0, "ierr": [], "mass_scale": []}
+    rec = {}
+    with torch.cuda.stream(cm.torch_stream()):
+        fluid = torch.as_tensor(cm.mask, dtype=torch.float32).cuda()
+        n_c = fluid.view(H // f, f, W // f, f).sum(dim=(1, 3))
+        solid_c = (1.0 - n_c / (f * f))[None]
+        fluid_c = (n_c > 0).float()
+        inflow = 2.0 * cm.params.u0 * float(fluid[:, 0].sum()) * jump
+        Mt = torch.tensor([M], dtype=torch.float32, device="cuda")
+        Tt = torch.tensor([theta_inv], dtype=torch.float32, device="cuda")
+        buf = buf2 = state = None
+        torch.cuda.current_stream().synchronize()
+        t0 = time.perf_counter()
+        cm.step(relax)
+        t_model = relax
+        t_b = None
+        while t_model < max_steps:
+            buf = cm.fields_device(buf)
+            rec[t_model] = coarsen(buf[0], fluid, f)
+            if bool(reached(buf, fluid, col)):
+                t_b = t_model
+                break
+            pc, uxc, uyc = coarsen(buf[[0, 2, 3]], fluid, f)
+            x = inputs(pc[None], uxc[None], uyc[None], solid_c, Mt, Tt)
+            pred = (pc + model(x)[0]).clamp(-1.0, 1.0)
+            delta = (pred - pc) * fluid_c
+            if mass_fix:
+                gain = (delta * n_c).sum()
+                scale = torch.where(gain > 0, inflow / gain.clamp(min=1e-6), torch.ones_like(gain)).clamp(0.5, 2.0)
+                delta = delta * scale
+                stats["mass_scale"].append(scale)
+            psi_new = torch.where(fluid > 0, (buf[0] + refine(delta, f)).clamp(-1.0, 1.0), buf[0])
+            state = cm.save_state_device(state[0] if state else None)
+            cm.init_fields_device(psi_new, buf[1], buf[2], buf[3])
+            cm.step(relax)
+            buf2 = cm.fields_device(buf2)
+            err = float(interface_error((pc + delta).clamp(-1.0, 1.0), coarsen(buf2[0], fluid, f), fluid_c))
+            stats["ierr"].append(err)
+            if err > threshold:
+                stats["fallbacks"] += 1
+                cm.load_state_device(state)
+                cm.step(jump + relax)
+            else:
+                stats["jumps"] += 1
+            t_model += jump + relax
+            if stats["jumps"] + stats["fallbacks"] == 1 or t_model % 100000 < jump + relax:
+                log("accelerated: t=%d jumps %d fallbacks %d ierr %.4f (%.1f s)" % (
+                    t_model, stats["jumps"], stats["fallbacks"], err, time.perf_counter() - t0))
+        torch.cuda.current_stream().synchronize()
+        wall = time.perf_counter() - t0
+    stats["ierr_mean"] = float(np.mean(stats["ierr"])) if stats["ierr"] else 0.0
+    stats["ierr_max"] = float(np.max(stats["ierr"])) if stats["ierr"] else 0.0
+    del stats["ierr"]
+    ms = [float(v) for v in stats.pop("mass_scale")]
+    stats["mass_scale_mean"] = float(np.mean(ms)) if ms else None
+    return t_b, wall, stats, rec

@@ -1,0 +1,170 @@
+"""The surrogate pipeline end to end on one GPU, as one job:
+
+1. trajectories of `--domains` flows (surrogate.flow_spec, H x W), coarsened
+   by f every `--snap` iterations (batched LBM, one launch per iteration);
+2. training of the U-Net on pairs (t, t + jump) of all but `--holdout`
+   flows (AdamW, batch `--bs`);
+3. evaluation on the held-out flows against persistence;
+4. the MFMA throughput of a training step at batch 64 on the 256 x 128
+   coarse grid of the 2048 x 1024 workload;
+5. the 2048 x 1024 breakthrough workload (surrogate.breakthrough_spec) run
+   to breakthrough by the plain LBM and by the surrogate-accelerated loop,
+   with wall times, breakthrough iterations and the coarse interface error
+   of the accelerated run against the plain one at equal iterations.
+
+    python -m fingering_dynamics_amd.surrogate_run --out DIR [options]
+
+Writes DIR/surrogate_run.json (every figure above) and DIR/unet.npz (the
+trained weights); --load-model NPZ skips steps 1-3.  `--mfma-only` runs
+step 4 alone (for profiling the convolution kernels).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import surrogate as sg
+from .lbm import Compute
+
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def save_model(model, path):
+    np.savez(path, **{k: v.detach().cpu().numpy() for k, v in model.state_dict().items()})
+
+
+def load_model(model, path):
+    with np.load(path, allow_pickle=False) as z:
+        model.load_state_dict({k: torch.as_tensor(z[k]) for k in z.files})
+    return model
+
+
+def mfma_throughput(model, H=128, W=256, bs=64, iters=20):
+    """Forward + backward of the U-Net on a random batch: TFLOP/s of the
+    convolutions (3x the forward FLOPs) against the bf16 dense peak."""
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn(bs, H, W, sg.IN_CH, device="cuda", generator=g).to(torch.bfloat16)
+    tgt = torch.randn(bs, H, W, device="cuda", generator=g)
+    for _ in range(3):
+        ((model(x) - tgt) ** 2).mean().backward()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        model.zero_grad(set_to_none=True)
+        ((model(x) - tgt) ** 2).mean().backward()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    flops = 3.0 * model.flops_per_sample(H, W) * bs
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"batch": bs, "H": H, "W": W, "ms_per_step": ms, "tflops": tf, "peak_tflops": BF16_PEAK_TFLOPS,
+            "frac": tf / BF16_PEAK_TFLOPS, "gflop_per_step": flops / 1e9}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--domains", type=int, default=64)
+    ap.add_argument("--H", type=int, default=256)
+    ap.add_argument("--W", type=int, default=512)
+    ap.add_argument("--f", type=int, default=8)
+    ap.add_argument("--snap", type=int, default=500)
+    ap.add_argument("--data-steps", type=int, default=200000)
+    ap.add_argument("--holdout", type=int, default=8)
+    ap.add_argument("--jump", type=int, default=2000)
+    ap.add_argument("--relax", type=int, default=200)
+    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--iters", type=int, default=4000)
+    ap.add_argument("--bs", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--max-steps", type=int, default=3000000)
+    ap.add_argument("--no-mass-fix", action="store_true")
+    ap.add_argument("--load-model")
+    ap.add_argument("--mfma-only", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true", help="accelerated run only")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    res = {"args": vars(a)}
+    model = sg.UNet(seed=0).cuda()
+    res["n_params"] = model.n_params()
+
+    if a.mfma_only:
+        res["mfma"] = mfma_throughput(model)
+        log(json.dumps(res["mfma"]))
+        return 0
+
+    if a.load_model:
+        load_model(model, a.load_model)
+    else:
+        log("generating %d flows %dx%d, %d iterations" % (a.domains, a.W, a.H, a.data_steps))
+        data = sg.generate(a.domains, a.H, a.W, a.f, a.snap, a.data_steps, log=log)
+        D = data["psi"].shape[0]
+        res["data"] = {"domains": D, "dropped": data["dropped"], "snapshots": len(data["steps"]),
+                       "seconds": data["seconds"], "mlups": data["lattice_updates"] / data["seconds"] / 1e6}
+        log(json.dumps(res["data"]))
+        hold = min(a.holdout, max(D // 4, 1))
+        tr, te = list(range(D - hold)), list(range(D - hold, D))
+        res["eval_untrained"] = sg.evaluate(model, data, te, a.jump, a.snap)
+        hist, secs = sg.train(model, data, tr, a.jump, a.snap, iters=a.iters, bs=a.bs, lr=a.lr, log=log)
+        res["train"] = {"iters": a.iters, "seconds": secs, "loss_first": hist[0][1], "loss_last": hist[-1][1]}
+        res["eval_train"] = sg.evaluate(model, data, tr[:hold], a.jump, a.snap)
+        res["eval_heldout"] = sg.evaluate(model, data, te, a.jump, a.snap)
+        log("held-out: %s" % json.dumps(res["eval_heldout"]))
+        save_model(model, os.path.join(a.out, "unet.npz"))
+        del data
+        torch.cuda.empty_cache()
+    res["mfma"] = mfma_throughput(model)
+    log("mfma: %s" % json.dumps(res["mfma"]))
+
+    spec = sg.breakthrough_spec()
+    mask, links, params = sg.build_flow(spec)
+    res["workload"] = dict(spec, porosity_reached=float(mask.mean()), psi_wall=params.psi_wall, u0=params.u0,
+                           probe_column=sg.probe_column(spec["W"]))
+    cycle = a.jump + a.relax
+    rec_p = {}
+    if not a.skip_plain:
+        cm = Compute(mask, links, params)
+        t_b, wall, rec_p = sg.plain_run(cm, a.f, a.max_steps, chunk=a.relax, record_every=cycle, log=log)
+        res["plain"] = {"breakthrough_t": t_b, "wall_s": wall, "mlups": cm.t * mask.size / wall / 1e6,
+                        "non_finite": cm.mass()[2]}
+        cm.close()
+        log("plain: %s" % json.dumps(res["plain"]))
+    cm = Compute(mask, links, params)
+    model.eval()
+    t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, model, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
+                                                   a.relax, a.threshold, a.max_steps,
+                                                   mass_fix=not a.no_mass_fix, log=log)
+    res["accelerated"] = dict(stats, breakthrough_t=t_a, wall_s=wall_a, non_finite=cm.mass()[2])
+    cm.close()
+    if rec_p:
+        fluid = torch.as_tensor(mask, dtype=torch.float32).cuda()
+        fluid_c = (fluid.view(spec["H"] // a.f, a.f, spec["W"] // a.f, a.f).sum(dim=(1, 3)) > 0).float()
+        common = sorted(set(rec_p) & set(rec_a))
+        errs = [float(sg.interface_error(rec_a[t], rec_p[t], fluid_c)) for t in common]
+        p = res["plain"]
+        res["comparison"] = {
+            "speedup_wall": p["wall_s"] / wall_a if wall_a > 0 else None,
+            "breakthrough_t_rel_error": (abs(t_a - p["breakthrough_t"]) / p["breakthrough_t"]
+                                         if t_a and p["breakthrough_t"] else None),
+            "interface_error_vs_plain_mean": float(np.mean(errs)) if errs else None,
+            "interface_error_vs_plain_last": errs[-1] if errs else None,
+            "compared_at": len(common),
+        }
+    log(json.dumps(res, indent=1, default=float))
+    with open(os.path.join(a.out, "surrogate_run.json"), "w") as fh:
+        json.dump(res, fh, indent=1, default=float)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -96,6 +96,13 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
  * the step counter to 0. */
 int fd_init(fd_handle* h, const double* psi0, const double* rho0);
 
+/* Restart from given fields, double[H*W] each: the equilibrium state of
+ * psi, rho and velocity (ux, uy) with mu and the force mu grad psi taken
+ * from psi, collided once - the state a running flow had, with a new phase
+ * field.  Resets the step counter to 0.  No reference counterpart: used
+ * where a learned surrogate hands the phase field back to the solver. */
+int fd_init_fields(fd_handle* h, const double* psi, const double* rho, const double* ux, const double* uy);
+
 /* Advance nsteps iterations of the main loop body
  * (fingering_periodic.py:454-479: collide, stream, bounce-back, walls,
  * Zou-He, moments).  Enqueued on the handle's stream; returns without
@@ -140,6 +147,26 @@ int fd_phase_mass(fd_handle* h, double out[3]);
 int64_t fd_state_bytes(const fd_handle* h);
 int fd_save_state(fd_handle* h, void* buf, int64_t nbytes);
 int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes);
+
+/* Device-array coupling (no reference counterpart: the reference keeps its
+ * fields in numpy, fingering_periodic.py:472-479).  These take DEVICE
+ * pointers, are enqueued on the handle's stream and return without
+ * waiting; a caller that produces or consumes the arrays on the GPU runs
+ * its own work on fd_stream(h) (a hipStream_t) to stay ordered.
+ *   fd_fields_device       psi, rho, ux, uy of the last completed iteration
+ *                          into float[H*W] device arrays (any may be NULL).
+ *   fd_init_fields_device  fd_init_fields from float[H*W] device arrays
+ *                          (all four required); the step count restarts at 0.
+ *   fd_state_device_bytes / fd_save_state_device / fd_load_state_device
+ *                          the resumable state (populations and Zou-He side
+ *                          buffer) as a device blob; the step count is kept
+ *                          by the caller and handed back on load (>= 1). */
+void* fd_stream(const fd_handle* h);
+int fd_fields_device(fd_handle* h, float* psi, float* rho, float* ux, float* uy);
+int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, const float* ux, const float* uy);
+int64_t fd_state_device_bytes(const fd_handle* h);
+int fd_save_state_device(fd_handle* h, void* dbuf, int64_t nbytes);
+int fd_load_state_device(fd_handle* h, const void* dbuf, int64_t nbytes, int64_t steps_done);
 
 /* Wait for the handle's queued work. */
 int fd_synchronize(fd_handle* h);

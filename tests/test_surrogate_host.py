@@ -1,0 +1,96 @@
+"""Host-side pieces of the surrogate (CPU tensors): coarsening and its
+inverse, the interface error, the flow generators, the network's parameter
+and FLOP counts, and the weight file round trip."""
+import math
+
+import numpy as np
+import torch
+
+from fingering_dynamics_amd import surrogate as sg
+
+
+def _fluid(H, W, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.random((H, W)) > 0.3
+    m[:8, :8] = False            # one all-solid 8 x 8 block
+    return torch.as_tensor(m, dtype=torch.float32)
+
+
+def test_coarsen_averages_fluid_nodes_and_conserves_the_sum():
+    H, W, f = 32, 48, 8
+    fl = _fluid(H, W, 1)
+    field = torch.as_tensor(np.random.default_rng(2).standard_normal((H, W)), dtype=torch.float32)
+    c = sg.coarsen(field, fl, f)
+    assert c.shape == (H // f, W // f)
+    assert float(c[0, 0]) == 0.0
+    # brute force on one block
+    blk = (slice(8, 16), slice(16, 24))
+    want = float((field[blk] * fl[blk]).sum() / fl[blk].sum())
+    assert abs(float(c[1, 2]) - want) < 1e-6
+    n = fl.view(H // f, f, W // f, f).sum(dim=(1, 3))
+    assert abs(float((c * n).sum()) - float((field * fl).sum())) < 1e-3
+    # batched leading dimensions give the same per-field result
+    cb = sg.coarsen(torch.stack([field, 2 * field]), fl, f)
+    assert torch.allclose(cb[0], c) and torch.allclose(cb[1], 2 * c)
+    sf = sg.solid_fraction(fl, f)
+    assert float(sf[0, 0]) == 1.0 and torch.allclose(sf, 1.0 - n / 64)
+
+
+def test_refine_inverts_coarsen_on_block_constant_fields():
+    H, W, f = 16, 32, 8
+    fl = _fluid(H, W, 3)
+    coarse = torch.arange(8, dtype=torch.float32).view(2, 4) - 3.5
+    fine = sg.refine(coarse, f)
+    assert fine.shape == (H, W)
+    c = sg.coarsen(fine, fl, f)
+    n = fl.view(2, f, 4, f).sum(dim=(1, 3))
+    assert torch.equal(c[n > 0], coarse[n > 0])
+
+
+def test_interface_error_counts_fluid_cells_of_other_sign():
+    a = torch.tensor([[1.0, -1.0], [0.5, -0.2]])
+    b = torch.tensor([[0.9, 0.3], [-0.5, -0.9]])
+    fc = torch.tensor([[1.0, 1.0], [1.0, 0.0]])
+    assert abs(float(sg.interface_error(a, b, fc)) - 2.0 / 3.0) < 1e-7
+    assert float(sg.interface_error(a, a, fc)) == 0.0
+
+
+def test_flow_specs():
+    s1, s2 = sg.flow_spec(5), sg.flow_spec(5)
+    assert s1 == s2
+    for k in range(50):
+        s = sg.flow_spec(k)
+        assert 0.6 <= s["porosity"] <= 0.8 and 1.0 <= s["M"] <= 20.0 and 30.0 <= s["theta"] <= 90.0
+    b = sg.breakthrough_spec()
+    assert (b["H"], b["W"], b["porosity"], b["M"], b["theta"]) == (1024, 2048, 0.7, 15.0, 45.0)
+    m, l, p = sg.build_flow(dict(sg.flow_spec(3), H=64, W=128))
+    assert m.shape == (64, 128) and l.shape == (64, 128)
+    assert abs(m.mean() - sg.flow_spec(3)["porosity"]) < 0.05
+    assert m[:, :16].all() and m[:, -16:].all()          # inlet and outlet kept clear
+    assert p.u0 == sg.U_IN and p.n_inv_cols == sg.INV_COLS
+    assert sg.probe_column(2048) == 2047 - 8
+
+
+def test_unet_counts():
+    net = sg.UNet(channels=(8, 16, 16, 16), seed=0)
+    c0, c1, c2, c3 = 8, 16, 16, 16
+    plan = [(8, c0), (c0, c0), (c0, c1), (c1, c1), (c1, c2), (c2, c2), (c2, c3), (c3, c3), (c3 + c2, c2), (c2, c2),
+            (c2 + c1, c1), (c1, c1), (c1 + c0, c0), (c0, c0), (c0, 8)]
+    assert net.n_params() == sum(9 * a * b + b for a, b in plan)
+    H, W = 16, 32
+    res = [1, 1, 2, 2, 4, 4, 8, 8, 4, 4, 2, 2, 1, 1, 1]
+    want = sum(2 * 9 * a * b * (H // r) * (W // r) for (a, b), r in zip(plan, res))
+    assert net.flops_per_sample(H, W) == want
+    full = sg.UNet()
+    assert 1.0e6 < full.n_params() < 1.5e6
+
+
+def test_model_file_round_trip(tmp_path):
+    from fingering_dynamics_amd.surrogate_run import save_model, load_model
+    a = sg.UNet(channels=(8, 8, 16, 16), seed=1)
+    b = sg.UNet(channels=(8, 8, 16, 16), seed=2)
+    path = str(tmp_path / "w.npz")
+    save_model(a, path)
+    load_model(b, path)
+    for (ka, va), (kb, vb) in zip(a.state_dict().items(), b.state_dict().items()):
+        assert ka == kb and torch.equal(va, vb)
