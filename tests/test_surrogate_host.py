@@ -63,11 +63,13 @@ def test_flow_specs():
         assert 0.6 <= s["porosity"] <= 0.8 and 1.0 <= s["M"] <= 20.0 and 30.0 <= s["theta"] <= 90.0
     b = sg.breakthrough_spec()
     assert (b["H"], b["W"], b["porosity"], b["M"], b["theta"]) == (1024, 2048, 0.7, 15.0, 45.0)
-    m, l, p = sg.build_flow(dict(sg.flow_spec(3), H=64, W=128))
-    assert m.shape == (64, 128) and l.shape == (64, 128)
-    assert abs(m.mean() - sg.flow_spec(3)["porosity"]) < 0.05
-    assert m[:, :16].all() and m[:, -16:].all()          # inlet and outlet kept clear
-    assert p.u0 == sg.U_IN and p.n_inv_cols == sg.INV_COLS
+    m, l, p = sg.build_flow(dict(sg.flow_spec(3), H=64, W=320))
+    assert m.shape == (64, 320) and l.shape == (64, 320)
+    bed = m[:, 64:256]
+    assert abs(bed.mean() - (1 - (1 - sg.flow_spec(3)["porosity"]) * 320 / 192)) < 0.08
+    assert m[:, :64].all() and m[:, -64:].all()          # inlet and outlet kept clear
+    assert m[:2].all() and m[-2:].all()                  # obstacles stay inside the rows (no wrap)
+    assert p.u0 == sg.U_IN and p.n_inv_cols == sg.INV_COLS and p.y_bc == 0
     assert sg.probe_column(2048) == 2047 - 8
 
 
