@@ -24,6 +24,7 @@ class FdParams(ctypes.Structure):
         ("y_bc", ctypes.c_int32), ("x_bc", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("outlet", ctypes.c_int32), ("init_mode", ctypes.c_int32),
         ("wall_lo", ctypes.c_int32), ("wall_hi", ctypes.c_int32),
+        ("psi_visc_bound", ctypes.c_double),
     ]
 
 

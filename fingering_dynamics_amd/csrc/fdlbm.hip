@@ -843,6 +843,7 @@ void fill_kparams(fd_handle* h) {
     k.c.kappa = (float)p.kappa;
     k.c.gamma = (float)p.gamma;
     k.c.psi_wall = (float)p.psi_wall;
+    k.c.psi_visc = p.psi_visc_bound > 0 ? (float)p.psi_visc_bound : 3.0e38f;
     k.c.gx = (float)p.gx;
     k.c.gy = (float)p.gy;
 }

@@ -44,62 +44,84 @@ __device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast
 // ---------------------------------------------------------------- forward
 // y[p][co] = act(bias[co] + sum_k A[p][k] w[co][k]), p over N*H*W (NHWC),
 // k = tap*Cin + ci with tap = 3*(dh+1) + (dw+1) and A[p][k] = x at pixel
-// (n, h+dh, w+dw), channel ci (0 outside the image).  One wave computes 16
-// consecutive pixels x 64 output channels (four 16-column MFMA tiles); a
-// workgroup of four waves covers 64 pixels.  The K loop takes 32 k per
-// step; k beyond 9*Cin and channels beyond Cout load zeros.
+// (n, h+dh, w+dw), channel ci (0 outside the image).  One wave computes
+// 16*MP consecutive pixels x 16*NT output channels (MP x NT MFMA tiles, so
+// every A fragment is used NT times and every B fragment MP times); the
+// four waves of a workgroup take consecutive pixel ranges with the same
+// channels, so they share the weight fragments through L1.  The K loop
+// takes 32 k per step (one tap and 8 channels per lane, a 16-byte load);
+// k beyond 9*Cin and channels beyond Cout load zeros.
+template <int MP, int NT>
 __global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                                                 const float* __restrict__ bias, float* __restrict__ y32,
                                                 uint16_t* __restrict__ y16, int N, int H, int W, int Cin, int Cout,
                                                 int relu) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t P = (int64_t)N * H * W;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
-    const int co0 = blockIdx.y * 64;
+    const int64_t pw = ((int64_t)blockIdx.x * 4 + wave) * (16 * MP);
+    const int co0 = blockIdx.y * (16 * NT);
     const int K = 9 * Cin;
-    const int64_t pa = p0 + (lane & 15);
-    const bool pin = pa < P;
-    int n = 0, h = 0, wc = 0;
-    if (pin) {
-        wc = (int)(pa % W);
-        const int64_t t = pa / W;
-        h = (int)(t % H);
-        n = (int)(t / H);
-    }
-    v4f acc[4];
+    int hq[MP], wq[MP];
+    int64_t img[MP];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < MP; ++g) {
+        const int64_t pa = pw + 16 * g + (lane & 15);
+        hq[g] = -4;  // out of range for every tap: pixels past the end load zeros
+        wq[g] = 0;
+        img[g] = 0;
+        if (pa < P) {
+            wq[g] = (int)(pa % W);
+            const int64_t t = pa / W;
+            hq[g] = (int)(t % H);
+            img[g] = (t / H) * H;
+        }
+    }
+    v4f acc[MP][NT];
+#pragma unroll
+    for (int g = 0; g < MP; ++g)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[g][t] = v4f{0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < K; kc += 32) {
         const int k0 = kc + 8 * (lane >> 4);
-        u4v av = {0u, 0u, 0u, 0u};
-        if (pin && k0 < K) {
-            const int tap = k0 / Cin, ci = k0 - tap * Cin;
-            const int hh = h + tap / 3 - 1, ww = wc + tap % 3 - 1;
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W)
-                av = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci);
+        const bool kin = k0 < K;
+        const int tap = k0 / Cin, ci = k0 - tap * Cin;
+        const int dh = tap / 3 - 1, dw = tap % 3 - 1;
+        u4v av[MP], bv[NT];
+#pragma unroll
+        for (int g = 0; g < MP; ++g) {
+            av[g] = u4v{0u, 0u, 0u, 0u};
+            const int hh = hq[g] + dh, ww = wq[g] + dw;
+            if (kin && hh >= 0 && hh < H && ww >= 0 && ww < W)
+                av[g] = *reinterpret_cast<const u4v*>(x + ((img[g] + hh) * W + ww) * Cin + ci);
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NT; ++t) {
             const int co = co0 + 16 * t + (lane & 15);
-            u4v bv = {0u, 0u, 0u, 0u};
-            if (co < Cout && k0 < K) bv = *reinterpret_cast<const u4v*>(w + (int64_t)co * K + k0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf8(av), bf8(bv), acc[t], 0, 0, 0);
+            bv[t] = u4v{0u, 0u, 0u, 0u};
+            if (co < Cout && kin) bv[t] = *reinterpret_cast<const u4v*>(w + (int64_t)co * K + k0);
         }
+#pragma unroll
+        for (int g = 0; g < MP; ++g)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf8(av[g]), bf8(bv[t]), acc[g][t], 0, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < NT; ++t) {
         const int co = co0 + 16 * t + (lane & 15);
         if (co >= Cout) continue;
         const float b = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t p = p0 + 4 * (lane >> 4) + r;
-            if (p >= P) continue;
-            float v = acc[t][r] + b;
-            if (relu) v = fmaxf(v, 0.f);
-            if (y32) y32[p * Cout + co] = v;
-            if (y16) y16[p * Cout + co] = to_bf16(v);
-        }
+        for (int g = 0; g < MP; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t p = pw + 16 * g + 4 * (lane >> 4) + r;
+                if (p >= P) continue;
+                float v = acc[g][t][r] + b;
+                if (relu) v = fmaxf(v, 0.f);
+                if (y32) y32[p * Cout + co] = v;
+                if (y16) y16[p * Cout + co] = to_bf16(v);
+            }
     }
 }
 
@@ -117,19 +139,26 @@ __global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__
 
 // ------------------------------------------------------- weight gradient
 // dw[co][tap][ci] = sum_p dy[p][co] x[p + d(tap)][ci]: a GEMM whose
-// reduction runs over pixels, which are the strided dimension of both
-// operands, so each 32-pixel step stages dy (32 x 64 channels) and the
-// shifted x (32 x 64 channels) in LDS and the fragments are read from there
-// (rows padded to 72 elements).  A workgroup owns one tap, 64 co x 64 ci and
-// one slice of pixels; wave w computes co rows 16w..16w+15.  Slices write
-// partial sums that reduce_slices adds in a fixed order.
-constexpr int kLds = 72;
+// reduction runs over pixels, the strided dimension of both operands.  Each
+// 64-pixel step loads dy (64 px x 64 co) and the shifted x (64 px x 64 ci)
+// with 16-byte loads (8 channels of one pixel) and stores them TRANSPOSED
+// in LDS, [channel][pixel] with rows padded to 72 elements, so that every
+// MFMA fragment (8 consecutive pixels of one channel) is one 16-byte LDS
+// read.  Lane l of wave v stages pixels (l & 31) and (l & 31) + 32 of
+// channel group (l >> 5) + 2v: for a fixed element the 64 lanes write two
+// rows 8 apart (32 banks apart) and 32 consecutive pixels, free of bank
+// conflicts.  A workgroup owns one tap, 64 co x 64 ci and one slice of
+// pixels; wave v computes co rows 16v..16v+15 (four 16 x 16 tiles, two
+// 32-pixel MFMA steps per stage).  Slices write partial sums that
+// reduce_slices adds in a fixed order.
+constexpr int kStep = 64;
+constexpr int kLds = kStep + 8;
 
 __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                                                   float* __restrict__ part, int N, int H, int W, int Cin, int Cout,
                                                   int64_t chunk) {
-    __shared__ __align__(16) uint16_t sdy[32][kLds];
-    __shared__ __align__(16) uint16_t sx[32][kLds];
+    __shared__ __align__(16) uint16_t sdy[64][kLds];
+    __shared__ __align__(16) uint16_t sx[64][kLds];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ncb = (Cout + 63) / 64;
     int gy = blockIdx.y;
@@ -144,31 +173,39 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     v4f acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int pr = threadIdx.x >> 3, cg = (threadIdx.x & 7) * 8;
-    for (int64_t pc = pbeg; pc < pend; pc += 32) {
-        const int64_t p = pc + pr;
-        u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u};
-        if (p < pend) {
-            if (co0 + cg < Cout) vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
-            const int wc = (int)(p % W);
-            const int64_t t = p / W;
-            const int h = (int)(t % H), n = (int)(t / H);
-            const int hh = h + dh, ww = wc + dw;
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W && ci0 + cg < Cin)
-                vx = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+    const int pl = lane & 31, cg = ((lane >> 5) + 2 * wave) * 8;
+    const bool co_in = co0 + cg < Cout, ci_in = ci0 + cg < Cin;
+    for (int64_t pc = pbeg; pc < pend; pc += kStep) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int pr = pl + 32 * half;
+            const int64_t p = pc + pr;
+            u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u};
+            if (p < pend) {
+                if (co_in) vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+                const int wc = (int)(p % W);
+                const int64_t t = p / W;
+                const int h = (int)(t % H), n = (int)(t / H);
+                const int hh = h + dh, ww = wc + dw;
+                if (ci_in && hh >= 0 && hh < H && ww >= 0 && ww < W)
+                    vx = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                sdy[cg + e][pr] = (uint16_t)(vdy[e >> 1] >> (16 * (e & 1)));
+                sx[cg + e][pr] = (uint16_t)(vx[e >> 1] >> (16 * (e & 1)));
+            }
         }
-        *reinterpret_cast<u4v*>(&sdy[pr][cg]) = vdy;
-        *reinterpret_cast<u4v*>(&sx[pr][cg]) = vx;
         __syncthreads();
-        v8bf a;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) a[e] = __builtin_bit_cast(__bf16, sdy[8 * (lane >> 4) + e][16 * wave + (lane & 15)]);
+        for (int kk = 0; kk < 2; ++kk) {
+            const int kp = 32 * kk + 8 * (lane >> 4);
+            const v8bf a = bf8(*reinterpret_cast<const u4v*>(&sdy[16 * wave + (lane & 15)][kp]));
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v8bf b;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) b[e] = __builtin_bit_cast(__bf16, sx[8 * (lane >> 4) + e][16 * t + (lane & 15)]);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            for (int t = 0; t < 4; ++t) {
+                const v8bf b = bf8(*reinterpret_cast<const u4v*>(&sx[16 * t + (lane & 15)][kp]));
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
@@ -184,15 +221,30 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     }
 }
 
-// per-slice sums of dy over pixels (bias gradient): bpart[s][co]
+// per-slice sums of dy over pixels (bias gradient): bpart[s][co].  For
+// each block of up to 256 channels, the workgroup's threads split into
+// 256 / nc interleaved pixel rows; the row sums are added in row order.
 __global__ __launch_bounds__(256) void bias_part(const uint16_t* __restrict__ dy, float* __restrict__ bpart, int64_t P,
                                                  int Cout, int64_t chunk) {
+    __shared__ float red[256];
     const int64_t pbeg = (int64_t)blockIdx.x * chunk;
     const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
-    for (int co = threadIdx.x; co < Cout; co += blockDim.x) {
+    for (int c0 = 0; c0 < Cout; c0 += 256) {
+        const int nc = std::min(256, Cout - c0);
+        const int rows = 256 / nc;
+        const int co = threadIdx.x % nc, r = threadIdx.x / nc;
         float s = 0.f;
-        for (int64_t p = pbeg; p < pend; ++p) s += (float)__builtin_bit_cast(__bf16, dy[p * Cout + co]);
-        bpart[(int64_t)blockIdx.x * Cout + co] = s;
+        if (r < rows)
+            for (int64_t p = pbeg + r; p < pend; p += rows)
+                s += (float)__builtin_bit_cast(__bf16, dy[p * Cout + c0 + co]);
+        red[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x < nc) {
+            float t = 0.f;
+            for (int q = 0; q < rows; ++q) t += red[q * nc + threadIdx.x];
+            bpart[(int64_t)blockIdx.x * Cout + c0 + threadIdx.x] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -213,10 +265,10 @@ int check(int N, int H, int W, int Cin, int Cout) {
     return FDS_OK;
 }
 
-// pixel slices of the weight gradient: 32-pixel steps, at most 256 slices
+// pixel slices of the weight gradient: whole 64-pixel steps, at most 256 slices
 void slices(int64_t P, int64_t* chunk, int* S) {
     int64_t c = (P + 255) / 256;
-    c = std::max<int64_t>((c + 31) / 32 * 32, 512);
+    c = std::max<int64_t>((c + kStep - 1) / kStep * kStep, 512);
     *chunk = c;
     *S = (int)((P + c - 1) / c);
 }
@@ -234,9 +286,18 @@ int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, flo
     if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
     if (!x || !w || (!y32 && !y16)) return fail(FDS_E_ARG, "NULL pointer");
     const int64_t P = (int64_t)N * H * W;
-    const dim3 grid((unsigned)((P + 63) / 64), (unsigned)((Cout + 63) / 64));
-    hipLaunchKernelGGL(conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, x, w, bias, y32, y16, N, H, W, Cin, Cout,
-                       relu);
+    hipStream_t st = (hipStream_t)stream;
+    // wave tiles: 64 px x 64 co, 128 px x 32 co, or 128 px x 16 co
+    if (Cout >= 64) {
+        const dim3 grid((unsigned)((P + 255) / 256), (unsigned)((Cout + 63) / 64));
+        hipLaunchKernelGGL((conv_fwd<4, 4>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
+    } else if (Cout >= 32) {
+        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 31) / 32));
+        hipLaunchKernelGGL((conv_fwd<8, 2>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
+    } else {
+        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 15) / 16));
+        hipLaunchKernelGGL((conv_fwd<8, 1>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
+    }
     FDS_HIP(hipGetLastError());
     return FDS_OK;
 }

@@ -60,6 +60,7 @@ struct Consts {
     T a, kappa, gamma;
     T psi_wall;
     T gx, gy;      // body force (0 in the reference)
+    T psi_visc;    // |psi| bound of the viscosity interpolation (huge: none)
 };
 
 // psi on the node and its 8 neighbours in D2Q9 order:
@@ -80,8 +81,13 @@ FD_HD T chem_pot(T psi, T lap, const Consts<T>& c) {
 template <typename T>
 FD_HD T inv_tau_mix(T rho, T psi, const Consts<T>& c) {
     // 2 nu1 nu2 / (nu1 (1-psi) + nu2 (1+psi)) with nu1 = eta/rho, nu2 = M eta/rho
-    // = 2 M eta / (rho ((1-psi) + M (1+psi)))
-    const T tau = T(6) * c.M * c.eta_n * rcp(rho * ((T(1) - psi) + c.M * (T(1) + psi))) + T(0.5);
+    // = 2 M eta / (rho ((1-psi) + M (1+psi))).  The denominator vanishes at
+    // psi = -(1+M)/(M-1) (-1.14 for M = 15), which overshoots past -1 near
+    // wetting solids reach; with c.psi_visc = 1 the interpolation uses psi
+    // clamped to the pure phases.  With the default huge bound both
+    // comparisons fail and psi passes unchanged.
+    const T pv = psi < -c.psi_visc ? -c.psi_visc : (psi > c.psi_visc ? c.psi_visc : psi);
+    const T tau = T(6) * c.M * c.eta_n * rcp(rho * ((T(1) - pv) + c.M * (T(1) + pv))) + T(0.5);
     return rcp(tau);
 }
 

@@ -18,11 +18,19 @@ fdn::Consts<double> consts_from(const double c[9]) {
     k.psi_wall = c[6];
     k.gx = c[7];
     k.gy = c[8];
+    k.psi_visc = 1e300;
     return k;
 }
 }  // namespace
 
 extern "C" {
+
+// 1 / tau_mix with the viscosity interpolation's |psi| bound (<= 0: none).
+double fdn_inv_tau_mix_f64(double rho, double psi, const double c[9], double bound) {
+    fdn::Consts<double> k = consts_from(c);
+    if (bound > 0) k.psi_visc = bound;
+    return fdn::inv_tau_mix(rho, psi, k);
+}
 
 // n[9] psi on the node and its neighbours (C E N W S NE NW SW SE) -> out[3] = dx, dy, lap
 void fdn_stencil_f64(const double n[9], double out[3]) { fdn::stencil(n, out[0], out[1], out[2]); }

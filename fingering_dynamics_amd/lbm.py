@@ -28,7 +28,7 @@ def _to_fd_params(p: LbmParams) -> _native.FdParams:
         H=p.H, W=p.W, tau=p.tau, eta_n=p.eta_n, M=p.M, a=p.a, kappa=p.kappa, gamma=p.gamma,
         psi_wall=p.psi_wall, u0=p.u0, rho_out=p.rho_out, outlet_f3_coef=p.outlet_f3_coef,
         gx=p.gx, gy=p.gy, y_bc=p.y_bc, x_bc=p.x_bc, profile=p.profile, outlet=p.outlet,
-        init_mode=p.init_mode, wall_lo=p.wall_lo, wall_hi=p.wall_hi_row)
+        init_mode=p.init_mode, wall_lo=p.wall_lo, wall_hi=p.wall_hi_row, psi_visc_bound=p.psi_visc_bound)
 
 
 def initial_psi(params: LbmParams, mask):

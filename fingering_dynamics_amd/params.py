@@ -62,6 +62,7 @@ class LbmParams:
     wall_lo: int = 0
     wall_hi: int = -1          # -1 -> H-1
     n_inv_cols: int = 5        # psi = +1 in the first columns (fingering_periodic.py:91)
+    psi_visc_bound: float = 0.0  # > 0: |psi| bound of the tau_mix interpolation (0: reference formula)
 
     @classmethod
     def periodic(cls, H, W, M=20.0, psi_wall=-0.5, Pe=15.0, **kw):

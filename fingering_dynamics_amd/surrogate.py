@@ -5,8 +5,8 @@ lattice-Boltzmann solver, and the surrogate-accelerated run.
 No reference counterpart (the reference has no learned model).
 
 Network.  Input channels at the coarse resolution: psi, ux / U, uy / U,
-solid fraction, ln M / ln 20, theta / 180 deg (6, padded to 8); output: the
-change of psi over K iterations, added to the input psi.  Four levels of
+solid fraction, ln M / ln 20, theta / 180 deg (6, padded to 8); output:
+the change over K iterations of psi, ux / U and uy / U.  Four levels of
 two 3x3 convolutions each (32, 64, 128, 128 channels), 2x2 average pooling
 down, nearest-neighbour upsampling and skip concatenation up, ReLU after
 every convolution but the head: about 1.2M parameters.  Every convolution
@@ -16,20 +16,22 @@ master weights FP32 with AdamW.  Pooling, upsampling, concatenation, the
 ReLU mask and the optimiser are PyTorch tensor operations (plumbing around
 the convolutions).
 
-Data.  Domains of the many-domain workload at a fixed size (H x W, default
-256 x 512), advanced together by one batched launch per iteration; every
-`snap` iterations each domain's fields are area-averaged over f x f blocks
-(fluid nodes only) to the coarse grid.  Pairs (t, t + K).  Domains that go
-non-finite are dropped.
+Data.  Seeded beds of circles (flow_spec, build_flow) at a fixed size,
+advanced together by one batched launch per iteration; every `snap`
+iterations each domain's fields are area-averaged over f x f blocks (fluid
+nodes only) to the coarse grid, on the GPU.  Pairs (t, t + K) among each
+domain's usable snapshots (up to breakthrough, all finite).
 
 Accelerated run.  From the LBM state at t: the surrogate predicts the coarse
-psi at t + K, which is upsampled and handed back to the solver with its
-current density and velocity (fd_init_fields); the LBM then relaxes R
-iterations, and the interface error - the fraction of coarse fluid cells
-whose phase (sign of psi) differs between the prediction and the relaxed
-field - is judged.  Above the threshold the jump is discarded: the state at
-t is restored and the plain LBM runs K + R iterations instead.  Breakthrough
-is checked after every cycle at the probe column (output.breakthrough).
+psi at t + K; the predicted change near the current interface is added to
+the fine psi and handed back to the solver with its current density and
+velocity (fd_init_fields_device); the LBM then relaxes R iterations, and
+the interface error - the fraction of coarse fluid cells whose phase (sign
+of psi) differs between the prediction and the relaxed field - is judged.
+Above the threshold the jump is discarded: the state at t is restored and
+the plain LBM runs K + R iterations instead.  Breakthrough is checked after
+every cycle at the probe column.  Every array stays on the device and
+every launch goes to the solver's stream.
 """
 import math
 import time
@@ -40,6 +42,7 @@ import torch
 from . import output, surrogate_ops as so
 
 U_REF = 1e-3          # velocity scale of the inputs (the workload's inlet velocity)
+LAMBDA_U = 0.01       # weight of the velocity change in the training loss
 CHANNELS = (32, 64, 128, 128)
 IN_CH = 8
 
@@ -100,7 +103,8 @@ class UNet(torch.nn.Module):
         return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32)
 
     def forward(self, x):
-        """x [N, H, W, 8] bf16 (H, W multiples of 8) -> d psi [N, H, W] fp32."""
+        """x [N, H, W, 8] bf16 (H, W multiples of 8) -> [N, H, W, 3] fp32:
+        the change over K iterations of psi, ux / U_REF and uy / U_REF."""
         s0 = self._c("e0b", self._c("e0a", x))
         s1 = self._c("e1b", self._c("e1a", _pool(s0)))
         s2 = self._c("e2b", self._c("e2a", _pool(s1)))
@@ -108,7 +112,7 @@ class UNet(torch.nn.Module):
         d = self._c("d2b", self._c("d2a", torch.cat([_up(b), s2], dim=3).contiguous()))
         d = self._c("d1b", self._c("d1a", torch.cat([_up(d), s1], dim=3).contiguous()))
         d = self._c("d0b", self._c("d0a", torch.cat([_up(d), s0], dim=3).contiguous()))
-        return self._c("head", d, relu=False, out_f32=True)[..., 0]
+        return self._c("head", d, relu=False, out_f32=True)[..., :3]
 
     def n_params(self):
         return sum(p.numel() for p in self.parameters())
@@ -172,14 +176,23 @@ U_IN = 1e-3           # constant Zou-He inlet velocity
 INV_COLS = 40         # columns of invading fluid at t = 0
 
 
-def flow_spec(k, seed=4, H=256, W=512):
-    """Flow k of the training set: a bed of circles with radius U{8..24}
-    at porosity U(0.6, 0.8), viscosity ratio M = exp(U(0, ln 20)), contact
-    angle U(30, 90) degrees through the defending fluid; obstacle seed
-    1000 * seed + k."""
+REGIMES = {
+    # porosity, ln M, contact angle (degrees through the defending fluid)
+    "wide": ((0.6, 0.8), (0.0, math.log(20.0)), (30.0, 90.0)),
+    "target": ((0.65, 0.75), (math.log(10.0), math.log(20.0)), (35.0, 55.0)),
+}
+
+
+def flow_spec(k, seed=4, H=256, W=512, regime="wide"):
+    """Flow k of the training set: a bed of circles with radius U{8..24},
+    porosity, viscosity ratio M (log-uniform) and contact angle drawn
+    uniformly from the regime's ranges ("wide": porosity 0.6-0.8, M 1-20,
+    30-90 degrees; "target": 0.65-0.75, M 10-20, 35-55 degrees, around the
+    breakthrough workload); obstacle seed 1000 * seed + k."""
+    (p0, p1), (m0, m1), (a0, a1) = REGIMES[regime]
     rng = np.random.default_rng([seed, k])
-    return dict(k=k, H=H, W=W, porosity=float(rng.uniform(0.6, 0.8)),
-                M=float(math.exp(rng.uniform(0.0, math.log(20.0)))), theta=float(rng.uniform(30.0, 90.0)),
+    return dict(k=k, H=H, W=W, porosity=float(rng.uniform(p0, p1)),
+                M=float(math.exp(rng.uniform(m0, m1))), theta=float(rng.uniform(a0, a1)),
                 obstacle_seed=1000 * seed + k)
 
 
@@ -203,7 +216,13 @@ def build_flow(spec):
     and the inlet then delivers ~4x its nominal flux; most such beds go
     non-finite within a few thousand iterations (the FP64 oracle does the
     same).  Periodic y has no corners: the measured inflow of psi is the
-    nominal 2 u0 H per iteration and |u| stays ~5 u0."""
+    nominal 2 u0 H per iteration and |u| stays ~5 u0.
+
+    The tau_mix interpolation uses psi clamped to [-1, 1] (psi_visc_bound):
+    near the wetting obstacles psi overshoots below -1, and at M = 15 the
+    reference formula's denominator vanishes at psi = -1.143, which the
+    2048 x 1024 bed reaches after ~80,000 iterations (psi_min -1.1442, then
+    non-finite); at high M the training beds fail the same way."""
     from . import geometry, wetting
     from .params import LbmParams, Y_PERIODIC, PROFILE_CONST, OUTLET_PRESSURE
     H, W = spec["H"], spec["W"]
@@ -215,7 +234,7 @@ def build_flow(spec):
     links = geometry.link_bits_complete(mask)
     params = LbmParams.periodic(H, W, M=spec["M"], psi_wall=wetting.psi_wall_for_angle(180.0 - spec["theta"]),
                                 u0=U_IN, profile=PROFILE_CONST, outlet=OUTLET_PRESSURE, rho_out=1.0,
-                                y_bc=Y_PERIODIC, n_inv_cols=INV_COLS)
+                                y_bc=Y_PERIODIC, n_inv_cols=INV_COLS, psi_visc_bound=1.0)
     return mask, links, params
 
 
@@ -229,19 +248,30 @@ def reached(fields, fluid, column, threshold=0.0):
     return ((fields[0][:, column] > threshold) & (fluid[:, column] > 0)).any()
 
 
+def status(fields, fluid, column):
+    """(breakthrough, non-finite) of device fields, one host round trip."""
+    bad = (~torch.isfinite(fields[0][fluid > 0])).any() | (~torch.isfinite(fields[1][fluid > 0])).any()
+    r, b = torch.stack([reached(fields, fluid, column), bad]).tolist()
+    return bool(r), bool(b)
+
+
 # ------------------------------------------------------------------ data
-def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4, log=print):
+def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4, regime="wide", log=print):
     """Coarse trajectories of n_domains flows (flow_spec), advanced together
     by one batched launch per iteration; every `snap` iterations each
     domain's psi, ux, uy are area-averaged over f x f blocks on the GPU (on
     the domain's own stream).  Returns a dict of device tensors: psi, ux,
-    uy [D][T][H/f][W/f], solid [D][H/f][W/f], M, theta [D], plus the
-    iteration of each snapshot.  Domains that go non-finite are dropped."""
+    uy [D][T][H/f][W/f], solid [D][H/f][W/f], M, theta [D], the iteration
+    of each snapshot, and per domain the number of leading snapshots that
+    are usable (`valid`): up to and including the first one at or past
+    breakthrough (after it the invading fluid meets the outlet, whose
+    column is held at psi = -1), and before any non-finite one.  Domains
+    with fewer than two usable snapshots are dropped."""
     from .ensemble import Ensemble
     from .lbm import Compute
     specs, cs = [], []
     for k in range(n_domains):
-        spec = flow_spec(k, seed=seed, H=H, W=W)
+        spec = flow_spec(k, seed=seed, H=H, W=W, regime=regime)
         m, l, p = build_flow(spec)
         specs.append(spec)
         cs.append(Compute(m, l, p))
@@ -250,6 +280,9 @@ def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4
     Hc, Wc = H // f, W // f
     steps = list(range(0, max_steps + 1, snap))
     X = torch.zeros((n_domains, len(steps), 3, Hc, Wc), dtype=torch.float32, device="cuda")
+    fin = torch.zeros((n_domains, len(steps)), dtype=torch.bool, device="cuda")
+    bt = torch.zeros((n_domains, len(steps)), dtype=torch.bool, device="cuda")
+    col = probe_column(W)
     fluids, bufs = [], []
     for c in cs:
         with torch.cuda.stream(c.torch_stream()):
@@ -264,17 +297,29 @@ def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4
             with torch.cuda.stream(c.torch_stream()):
                 bufs[i] = c.fields_device(bufs[i])
                 X[i, j] = coarsen(bufs[i][[0, 2, 3]], fluids[i], f)
+                fin[i, j] = torch.isfinite(X[i, j]).all()
+                bt[i, j] = reached(bufs[i], fluids[i], col)
         if j % 40 == 0:
             log("data: t=%d (%.1f s)" % (t, time.perf_counter() - t0))
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    ok = [i for i, c in enumerate(cs) if c.mass()[2] == 0 and bool(torch.isfinite(X[i]).all())]
-    X = X[ok]
+    T = len(steps)
+    fin_h, bt_h = fin.cpu().numpy(), bt.cpu().numpy()
+    valid = []
+    for i in range(n_domains):
+        v = int(np.argmin(fin_h[i])) if not fin_h[i].all() else T
+        if bt_h[i].any():
+            v = min(v, int(np.argmax(bt_h[i])) + 1)
+        valid.append(v)
+    ok = [i for i in range(n_domains) if valid[i] >= 2]
+    X = torch.nan_to_num(X[ok], nan=0.0, posinf=0.0, neginf=0.0)
     data = {"psi": X[:, :, 0].contiguous(), "ux": X[:, :, 1].contiguous(), "uy": X[:, :, 2].contiguous(),
             "solid": torch.stack([solid_fraction(fluids[i], f) for i in ok]),
             "M": torch.tensor([specs[i]["M"] for i in ok], dtype=torch.float32, device="cuda"),
             "theta": torch.tensor([specs[i]["theta"] for i in ok], dtype=torch.float32, device="cuda"),
             "steps": steps, "specs": [specs[i] for i in ok], "dropped": n_domains - len(ok), "f": f,
+            "valid": [valid[i] for i in ok], "non_finite": int(sum(not fin_h[i].all() for i in range(n_domains))),
+            "broke_through": int(bt_h.any(axis=1).sum()),
             "seconds": gen_s, "lattice_updates": n_domains * H * W * max_steps}
     ens.close()
     for c in cs:
@@ -283,31 +328,43 @@ def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4
 
 
 def pairs(data, idx, jump, snap):
-    """All (domain, t) index pairs of domains idx with t + jump in range."""
+    """All (domain, t) index pairs of domains idx with t + jump among the
+    domain's usable snapshots."""
     d = jump // snap
     T = data["psi"].shape[1]
-    return [(i, t) for i in idx for t in range(T - d)], d
+    valid = data.get("valid", [T] * data["psi"].shape[0])
+    return [(i, t) for i in idx for t in range(valid[i] - d)], d
 
 
 def batch(data, items, d, flip=None):
-    """Network input, psi(t), psi(t + K) and fluid weight of (domain, t)
-    items; flip (bool tensor [B]) mirrors samples about the channel's
-    mid-plane (y -> H - 1 - y, uy -> -uy), a symmetry of the walled flow."""
+    """Network input, state at t and at t + K (psi [B, H, W] and velocity /
+    U_REF [B, H, W, 2]) and fluid weight of (domain, t) items; flip (bool
+    tensor [B]) mirrors samples about the channel's mid-plane
+    (y -> H - 1 - y, uy -> -uy), a symmetry of the y-periodic bed flow
+    statistics."""
     i = torch.tensor([a for a, _ in items], device="cuda")
     t = torch.tensor([b for _, b in items], device="cuda")
     p0, p1 = data["psi"][i, t], data["psi"][i, t + d]
-    ux, uy, sol = data["ux"][i, t], data["uy"][i, t], data["solid"][i]
+    ux0, uy0, ux1, uy1 = data["ux"][i, t], data["uy"][i, t], data["ux"][i, t + d], data["uy"][i, t + d]
+    sol = data["solid"][i]
     if flip is not None:
         fl = flip.view(-1, 1, 1)
-        p0, p1, ux, sol = [torch.where(fl, v.flip(-2), v) for v in (p0, p1, ux, sol)]
-        uy = torch.where(fl, -uy.flip(-2), uy)
-    x = inputs(p0, ux, uy, sol, data["M"][i], 180.0 - data["theta"][i])
-    return x, p0, p1, 1.0 - sol
+        p0, p1, ux0, ux1, sol = [torch.where(fl, v.flip(-2), v) for v in (p0, p1, ux0, ux1, sol)]
+        uy0, uy1 = [torch.where(fl, -v.flip(-2), v) for v in (uy0, uy1)]
+    x = inputs(p0, ux0, uy0, sol, data["M"][i], 180.0 - data["theta"][i])
+    v0 = torch.stack([ux0, uy0], dim=-1) / U_REF
+    v1 = torch.stack([ux1, uy1], dim=-1) / U_REF
+    return x, p0, p1, v0, v1, 1.0 - sol
 
 
-def loss_fn(model, x, psi0, psi1, wfluid):
-    pred = psi0 + model(x)
-    return ((pred - psi1) ** 2 * wfluid).sum() / wfluid.sum().clamp(min=1.0)
+def loss_fn(model, x, psi0, psi1, v0, v1, wfluid):
+    """Fluid-weighted MSE of psi(t + K) plus LAMBDA_U times that of the
+    velocity / U_REF."""
+    out = model(x)
+    w = wfluid.sum().clamp(min=1.0)
+    lp = ((psi0 + out[..., 0] - psi1) ** 2 * wfluid).sum() / w
+    lu = (((v0 + out[..., 1:3] - v1) ** 2).sum(dim=-1) * wfluid).sum() / w
+    return lp + LAMBDA_U * lu
 
 
 def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-4, seed=0, log=print):
@@ -323,15 +380,14 @@ def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-
     for it in range(iters):
         sel = [items[j] for j in rng.integers(0, len(items), bs)]
         flip = torch.rand(bs, device="cuda", generator=gen) < 0.5
-        x, p0, p1, wf = batch(data, sel, d, flip)
-        loss = loss_fn(model, x, p0, p1, wf)
+        loss = loss_fn(model, *batch(data, sel, d, flip))
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
         sched.step()
         if it % 100 == 0 or it == iters - 1:
-            hist.append((it, float(loss)))
-            log("train it %d loss %.3e (%.1f s)" % (it, float(loss), time.perf_counter() - t0))
+            hist.append((it, float(loss.detach())))
+            log("train it %d loss %.3e (%.1f s)" % (it, hist[-1][1], time.perf_counter() - t0))
     torch.cuda.synchronize()
     return hist, time.perf_counter() - t0
 
@@ -339,18 +395,22 @@ def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-
 @torch.no_grad()
 def evaluate(model, data, idx, jump, snap, bs=64):
     """Mean fluid-weighted MSE of the surrogate and of persistence (psi(t+K)
-    = psi(t), the method without the network), and the interface error of
-    both, over all pairs of domains idx."""
+    = psi(t), the method without the network), the interface error of
+    both, and the same MSE pair for the velocity / U_REF, over all pairs of
+    domains idx."""
     items, d = pairs(data, idx, jump, snap)
-    acc = {"mse": 0.0, "mse_persist": 0.0, "ierr": 0.0, "ierr_persist": 0.0}
+    acc = {"mse": 0.0, "mse_persist": 0.0, "ierr": 0.0, "ierr_persist": 0.0, "mse_u": 0.0, "mse_u_persist": 0.0}
     n = 0
     for k in range(0, len(items), bs):
         sel = items[k:k + bs]
-        x, p0, p1, wf = batch(data, sel, d)
-        pred = (p0 + model(x)).clamp(-1, 1)
+        x, p0, p1, v0, v1, wf = batch(data, sel, d)
+        out = model(x)
+        pred = (p0 + out[..., 0]).clamp(-1, 1)
         w = wf.sum().clamp(min=1.0)
         acc["mse"] += float(((pred - p1) ** 2 * wf).sum() / w) * len(sel)
         acc["mse_persist"] += float(((p0 - p1) ** 2 * wf).sum() / w) * len(sel)
+        acc["mse_u"] += float((((v0 + out[..., 1:3] - v1) ** 2).sum(dim=-1) * wf).sum() / w) * len(sel)
+        acc["mse_u_persist"] += float((((v0 - v1) ** 2).sum(dim=-1) * wf).sum() / w) * len(sel)
         acc["ierr"] += float(interface_error(pred, p1, wf)) * len(sel)
         acc["ierr_persist"] += float(interface_error(p0, p1, wf)) * len(sel)
         n += len(sel)
@@ -378,7 +438,11 @@ def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
             buf = cm.fields_device(buf)
             if record_every and (cm.t - chunk) % record_every == 0:
                 rec[cm.t] = coarsen(buf[0], fluid, f)
-            if bool(reached(buf, fluid, col)):
+            hit, bad = status(buf, fluid, col)
+            if bad:
+                log("plain: non-finite at t=%d" % cm.t)
+                break
+            if hit:
                 t_b = cm.t
                 break
             if cm.t % 100000 == 0:
@@ -389,7 +453,7 @@ def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
 
 @torch.no_grad()
 def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=0.02, max_steps=10 ** 7,
-                    mass_fix=True, log=print):
+                    mass_fix=True, velocity=0.0, band=2, log=print):
     """Surrogate-accelerated run of solver cm (freshly initialised) until
     breakthrough, every array on the device and every launch on the
     solver's stream.
@@ -397,9 +461,15 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
     A cycle: the coarse fields (psi, u) go through the network, which
     predicts the coarse psi K = jump iterations later; the predicted change
     is added to the fine psi (nearest-neighbour upsampling, clamped to
-    [-1, 1]), optionally rescaled so that the fluid sum of psi grows by the
+    [-1, 1]) within `band` coarse cells of the current interface (cells with
+    |psi| < 0.9; the front moves about 2 coarse cells per 2000 iterations,
+    and changes predicted farther away are noise that would seed islands of
+    either phase; 0: everywhere), optionally rescaled so that the fluid sum
+    of psi grows by the
     inflow's 2 u0 n_in K (invading fluid in, defending fluid out); the
-    solver restarts from (new psi, current rho, u) (fd_init_fields_device)
+    predicted coarse velocity change, times `velocity` (0: none), is added
+    likewise to the fluid nodes' velocity; the solver restarts from (new
+    psi, current rho, new u) (fd_init_fields_device)
     and relaxes `relax` iterations; the interface error between the
     prediction and the relaxed coarse psi is judged, and above `threshold`
     the state before the jump is restored and the plain LBM runs K + relax
@@ -427,21 +497,36 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
         while t_model < max_steps:
             buf = cm.fields_device(buf)
             rec[t_model] = coarsen(buf[0], fluid, f)
-            if bool(reached(buf, fluid, col)):
+            hit, bad = status(buf, fluid, col)
+            if bad:
+                stats["non_finite_at"] = t_model
+                log("accelerated: non-finite at t=%d" % t_model)
+                break
+            if hit:
                 t_b = t_model
                 break
             pc, uxc, uyc = coarsen(buf[[0, 2, 3]], fluid, f)
             x = inputs(pc[None], uxc[None], uyc[None], solid_c, Mt, Tt)
-            pred = (pc + model(x)[0]).clamp(-1.0, 1.0)
+            out = model(x)[0]
+            pred = (pc + out[..., 0]).clamp(-1.0, 1.0)
             delta = (pred - pc) * fluid_c
+            if band > 0:
+                near = (pc.abs() < 0.9).float()[None, None]
+                near = torch.nn.functional.max_pool2d(near, 2 * band + 1, stride=1, padding=band)[0, 0]
+                delta = delta * near
             if mass_fix:
                 gain = (delta * n_c).sum()
                 scale = torch.where(gain > 0, inflow / gain.clamp(min=1e-6), torch.ones_like(gain)).clamp(0.5, 2.0)
                 delta = delta * scale
                 stats["mass_scale"].append(scale)
             psi_new = torch.where(fluid > 0, (buf[0] + refine(delta, f)).clamp(-1.0, 1.0), buf[0])
+            if velocity:
+                ux_new = buf[2] + refine(out[..., 1] * fluid_c, f) * (velocity * U_REF * fluid)
+                uy_new = buf[3] + refine(out[..., 2] * fluid_c, f) * (velocity * U_REF * fluid)
+            else:
+                ux_new, uy_new = buf[2], buf[3]
             state = cm.save_state_device(state[0] if state else None)
-            cm.init_fields_device(psi_new, buf[1], buf[2], buf[3])
+            cm.init_fields_device(psi_new, buf[1], ux_new.contiguous(), uy_new.contiguous())
             cm.step(relax)
             buf2 = cm.fields_device(buf2)
             err = float(interface_error((pc + delta).clamp(-1.0, 1.0), coarsen(buf2[0], fluid, f), fluid_c))

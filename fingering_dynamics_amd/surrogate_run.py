@@ -53,7 +53,7 @@ def mfma_throughput(model, H=128, W=256, bs=64, iters=20):
     convolutions (3x the forward FLOPs) against the bf16 dense peak."""
     g = torch.Generator(device="cuda").manual_seed(0)
     x = torch.randn(bs, H, W, sg.IN_CH, device="cuda", generator=g).to(torch.bfloat16)
-    tgt = torch.randn(bs, H, W, device="cuda", generator=g)
+    tgt = torch.randn(bs, H, W, 3, device="cuda", generator=g)
     for _ in range(3):
         ((model(x) - tgt) ** 2).mean().backward()
     torch.cuda.synchronize()
@@ -74,17 +74,17 @@ def mfma_throughput(model, H=128, W=256, bs=64, iters=20):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--domains", type=int, default=64)
+    ap.add_argument("--domains", type=int, default=48)
     ap.add_argument("--H", type=int, default=256)
-    ap.add_argument("--W", type=int, default=512)
+    ap.add_argument("--W", type=int, default=2048)
     ap.add_argument("--f", type=int, default=8)
     ap.add_argument("--snap", type=int, default=500)
-    ap.add_argument("--data-steps", type=int, default=200000)
+    ap.add_argument("--data-steps", type=int, default=320000)
     ap.add_argument("--holdout", type=int, default=8)
     ap.add_argument("--jump", type=int, default=2000)
     ap.add_argument("--relax", type=int, default=200)
     ap.add_argument("--threshold", type=float, default=0.02)
-    ap.add_argument("--iters", type=int, default=4000)
+    ap.add_argument("--iters", type=int, default=8000)
     ap.add_argument("--bs", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--max-steps", type=int, default=3000000)
@@ -92,6 +92,12 @@ def main(argv=None):
     ap.add_argument("--load-model")
     ap.add_argument("--mfma-only", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="accelerated run only")
+    ap.add_argument("--variants", default="",
+                    help="further accelerated runs as relax:velocity-scale:band, e.g. 500:0:2,1000:0.5:2")
+    ap.add_argument("--band", type=int, default=2, help="coarse cells around the interface a jump may change")
+    ap.add_argument("--regime", default="target", choices=sorted(sg.REGIMES))
+    ap.add_argument("--velocity", type=float, default=0.0,
+                    help="scale of the predicted velocity change applied at each jump (0: keep the solver's)")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     res = {"args": vars(a)}
@@ -107,9 +113,11 @@ def main(argv=None):
         load_model(model, a.load_model)
     else:
         log("generating %d flows %dx%d, %d iterations" % (a.domains, a.W, a.H, a.data_steps))
-        data = sg.generate(a.domains, a.H, a.W, a.f, a.snap, a.data_steps, log=log)
+        data = sg.generate(a.domains, a.H, a.W, a.f, a.snap, a.data_steps, regime=a.regime, log=log)
         D = data["psi"].shape[0]
         res["data"] = {"domains": D, "dropped": data["dropped"], "snapshots": len(data["steps"]),
+                       "non_finite": data["non_finite"], "broke_through": data["broke_through"],
+                       "usable_snapshots": int(sum(data["valid"])),
                        "seconds": data["seconds"], "mlups": data["lattice_updates"] / data["seconds"] / 1e6}
         log(json.dumps(res["data"]))
         hold = min(a.holdout, max(D // 4, 1))
@@ -139,27 +147,36 @@ def main(argv=None):
                         "non_finite": cm.mass()[2]}
         cm.close()
         log("plain: %s" % json.dumps(res["plain"]))
-    cm = Compute(mask, links, params)
     model.eval()
-    t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, model, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
-                                                   a.relax, a.threshold, a.max_steps,
-                                                   mass_fix=not a.no_mass_fix, log=log)
-    res["accelerated"] = dict(stats, breakthrough_t=t_a, wall_s=wall_a, non_finite=cm.mass()[2])
-    cm.close()
-    if rec_p:
-        fluid = torch.as_tensor(mask, dtype=torch.float32).cuda()
-        fluid_c = (fluid.view(spec["H"] // a.f, a.f, spec["W"] // a.f, a.f).sum(dim=(1, 3)) > 0).float()
-        common = sorted(set(rec_p) & set(rec_a))
-        errs = [float(sg.interface_error(rec_a[t], rec_p[t], fluid_c)) for t in common]
-        p = res["plain"]
-        res["comparison"] = {
-            "speedup_wall": p["wall_s"] / wall_a if wall_a > 0 else None,
-            "breakthrough_t_rel_error": (abs(t_a - p["breakthrough_t"]) / p["breakthrough_t"]
-                                         if t_a and p["breakthrough_t"] else None),
-            "interface_error_vs_plain_mean": float(np.mean(errs)) if errs else None,
-            "interface_error_vs_plain_last": errs[-1] if errs else None,
-            "compared_at": len(common),
-        }
+    fluid = torch.as_tensor(mask, dtype=torch.float32).cuda()
+    fluid_c = (fluid.view(spec["H"] // a.f, a.f, spec["W"] // a.f, a.f).sum(dim=(1, 3)) > 0).float()
+    runs = [(a.relax, a.velocity, a.band)]
+    for v in a.variants.split(","):
+        if v:
+            r, s, b = (v.split(":") + [str(a.band)])[:3]
+            runs.append((int(r), float(s), int(b)))
+    for relax, vel, band in runs:
+        # the plain run's coarse snapshots sit at chunk + k * (jump + relax)
+        # only for the main relax length; other variants compare at breakthrough
+        cm = Compute(mask, links, params)
+        t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, model, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
+                                                       relax, a.threshold, a.max_steps,
+                                                       mass_fix=not a.no_mass_fix, velocity=vel, band=band, log=log)
+        acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
+                   non_finite=cm.mass()[2])
+        cm.close()
+        if "plain" in res:
+            p = res["plain"]
+            acc["speedup_wall"] = p["wall_s"] / wall_a if wall_a > 0 else None
+            acc["breakthrough_t_rel_error"] = (abs(t_a - p["breakthrough_t"]) / p["breakthrough_t"]
+                                               if t_a and p["breakthrough_t"] else None)
+            common = sorted(set(rec_p) & set(rec_a))
+            errs = [float(sg.interface_error(rec_a[t], rec_p[t], fluid_c)) for t in common]
+            acc["interface_error_vs_plain_mean"] = float(np.mean(errs)) if errs else None
+            acc["interface_error_vs_plain_last"] = errs[-1] if errs else None
+            acc["compared_at"] = len(common)
+        log("accelerated: %s" % json.dumps(acc))
+        res.setdefault("accelerated", []).append(acc)
     log(json.dumps(res, indent=1, default=float))
     with open(os.path.join(a.out, "surrogate_run.json"), "w") as fh:
         json.dump(res, fh, indent=1, default=float)

@@ -77,6 +77,8 @@ typedef struct fd_params {
     int32_t init_mode;     /* FD_INIT_*                                         */
     int32_t wall_lo;       /* FD_Y_WALLS: row of the bottom wall (0, or 1 as fingering.py:573 ships) */
     int32_t wall_hi;       /* FD_Y_WALLS: row of the top wall (H-1, or H-2)     */
+    double psi_visc_bound; /* > 0: |psi| <= b in the tau_mix interpolation    */
+                           /* (build-original), 0: fingering_periodic.py:201-208 */
 } fd_params;
 
 typedef struct fd_handle fd_handle;

@@ -41,6 +41,7 @@ typedef struct {
     int pressure;    /* outlet: 1 prescribed density, 0 velocity */
     int init_walled; /* 1: mu and uy at init (fingering.py:121-127) */
     int wall_lo, wall_hi;
+    double psi_visc_bound; /* > 0: psi clamped to [-b, b] in mix_tau (build-original option) */
 } o64_params;
 
 typedef struct {
@@ -98,6 +99,8 @@ static real_t mu_plain(const o64* o, size_t n, real_t psi) { /* :146-149 */
 
 static real_t mix_tau(const o64* o, real_t rho, real_t psi) { /* :201-208 */
     real_t v1 = o->p.eta_n / rho, v2 = o->p.eta_n * o->p.M / rho;
+    const real_t b = (real_t)o->p.psi_visc_bound;
+    if (b > 0) psi = psi < -b ? -b : (psi > b ? b : psi);
     return 3 * (2 * v1 * v2 / (v1 * (1 - psi) + v2 * (1 + psi))) + 0.5;
 }
 

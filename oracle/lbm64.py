@@ -116,8 +116,9 @@ class Lbm64:
                  rho0=1.0, n_inv_cols=5, inlet="gauss", outlet="velocity",
                  outlet_f3_coef=2.0 / 3.0, y_walls=False, x_periodic=False,
                  init="mu0", body_force=(0.0, 0.0), rho_out=1.0, psi0=None, rho_init=None,
-                 wall_rows="edge"):
+                 wall_rows="edge", psi_visc_bound=0.0):
         H, W = mask.shape
+        self.psi_visc_bound = float(psi_visc_bound)
         self.H, self.W = H, W
         self.mask = mask
         self.block_mask = np.logical_not(mask)
@@ -246,7 +247,11 @@ class Lbm64:
         ones = np.ones((self.H, self.W))[self.mask]
         v1 = self.eta_n / self.rho
         v2 = self.eta_n * self.M / self.rho
-        mix_v = np.divide(2 * v1 * v2, (v1 * (ones - self.psi[self.mask]) + v2 * (ones + self.psi[self.mask])))
+        psi = self.psi[self.mask]
+        if self.psi_visc_bound > 0:
+            # build-original option: the interpolation sees psi in [-b, b]
+            psi = np.clip(psi, -self.psi_visc_bound, self.psi_visc_bound)
+        mix_v = np.divide(2 * v1 * v2, (v1 * (ones - psi) + v2 * (ones + psi)))
         return 3 * mix_v + 0.5
 
     def udpatePsi(self):                                      # :210-212
@@ -469,7 +474,8 @@ class Lbm64:
             outlet_f3_coef=p.outlet_f3_coef, y_walls=p.y_bc == 1, x_periodic=p.x_bc == 1,
             init="walled" if p.init_mode == 1 else "mu0", body_force=(p.gx, p.gy),
             rho_out=p.rho_out, psi0=psi0, rho_init=rho0,
-            wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped")
+            wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped",
+            psi_visc_bound=getattr(p, "psi_visc_bound", 0.0))
 
     # ---------------------------------------------------------- outputs
     def field(self, name):

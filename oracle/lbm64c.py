@@ -15,7 +15,8 @@ class _P(ctypes.Structure):
         [(n, ctypes.c_double) for n in ("tau", "eta_n", "M", "a", "kappa", "gamma", "psi_wall", "u0",
                                         "rho_out", "c3", "gx", "gy", "rho0")] + \
         [(n, ctypes.c_int) for n in ("y_walls", "x_periodic", "gauss", "pressure", "init_walled",
-                                     "wall_lo", "wall_hi")]
+                                     "wall_lo", "wall_hi")] + \
+        [("psi_visc_bound", ctypes.c_double)]
 
 
 _libs = {}
@@ -52,7 +53,8 @@ class Lbm64C:
 
     def __init__(self, mask, links, *, bits=64, tau, eta_n, M, kappa, a, gamma, u0, psi_wall, rho0=1.0, n_inv_cols=5,
                  inlet="gauss", outlet="velocity", outlet_f3_coef=2.0 / 3.0, y_walls=False, x_periodic=False,
-                 init="mu0", body_force=(0.0, 0.0), rho_out=1.0, psi0=None, rho_init=None, wall_rows="edge"):
+                 init="mu0", body_force=(0.0, 0.0), rho_out=1.0, psi0=None, rho_init=None, wall_rows="edge",
+                 psi_visc_bound=0.0):
         lib = self._lib = _load(bits)
         H, W = mask.shape
         self.H, self.W = H, W
@@ -66,7 +68,8 @@ class Lbm64C:
         p = _P(H=H, W=W, tau=tau, eta_n=eta_n, M=M, a=a, kappa=kappa, gamma=gamma, psi_wall=psi_wall, u0=u0,
                rho_out=rho_out, c3=outlet_f3_coef, gx=body_force[0], gy=body_force[1], rho0=rho0,
                y_walls=int(y_walls), x_periodic=int(x_periodic), gauss=int(inlet == "gauss"),
-               pressure=int(outlet == "pressure"), init_walled=int(init == "walled"), wall_lo=lo, wall_hi=hi)
+               pressure=int(outlet == "pressure"), init_walled=int(init == "walled"), wall_lo=lo, wall_hi=hi,
+               psi_visc_bound=psi_visc_bound)
         fl = np.ascontiguousarray(self.mask, dtype=np.uint8)
         lk = np.ascontiguousarray(links, dtype=np.uint8)
         self._h = lib.o64_create(ctypes.byref(p), _ptr(fl), _ptr(lk), _ptr(psi0),
@@ -82,7 +85,8 @@ class Lbm64C:
                    outlet_f3_coef=p.outlet_f3_coef, y_walls=p.y_bc == 1, x_periodic=p.x_bc == 1,
                    init="walled" if p.init_mode == 1 else "mu0", body_force=(p.gx, p.gy), rho_out=p.rho_out,
                    psi0=psi0, rho_init=rho0,
-                   wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped")
+                   wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped",
+                   psi_visc_bound=getattr(p, "psi_visc_bound", 0.0))
 
     def step(self, n=1):
         self._lib.o64_step(self._h, int(n))

@@ -78,6 +78,10 @@ CASES = {
     "walled_pressure": dict(kind="circle", walled=True, edge=True,
                             params=dict(M=10.0, psi_wall=0.0, outlet=OUTLET_PRESSURE,
                                         outlet_f3_coef=2.0 / 3.0, gamma=1.0027)),
+    # the surrogate workload's configuration: M = 15, psi_wall of 45 degrees,
+    # constant inlet, pressure outlet, |psi| bound 1 in the tau_mix interpolation
+    "visc_bound": dict(kind="circle", params=dict(M=15.0, psi_wall=-0.525, profile=PROFILE_CONST,
+                                                  outlet=OUTLET_PRESSURE, rho_out=1.0, psi_visc_bound=1.0)),
     "periodic_x_force": dict(kind="rect", walled=True, edge=True,
                              params=dict(M=5.0, psi_wall=0.0, x_bc=X_PERIODIC, gx=1e-5, n_inv_cols=12,
                                          gamma=1.0027)),
@@ -235,3 +239,18 @@ def test_closed_domain_conservation_full_size():
     assert abs(r1 - r0) < 1e-5 * abs(r0)
     assert abs(p1 - p0) < 1e-5 * H * W
     cm.close()
+
+
+def test_parity_visc_bound_overshoot():
+    """psi = -1.25 (past the tau_mix pole of M = 15) on a few nodes: with the
+    |psi| bound the kernel follows the bounded oracle."""
+    H, W = 34, 70
+    mask, bits, links = rect_domain(H, W, 6, size=4, seed=7, x_min=14)
+    params = LbmParams.periodic(H, W, M=15.0, psi_wall=-0.525, profile=PROFILE_CONST, outlet=OUTLET_PRESSURE,
+                                psi_visc_bound=1.0)
+    psi0 = np.where(mask, -1.0, params.psi_wall)
+    psi0[:, :params.n_inv_cols] = 1.0
+    patch = np.zeros_like(mask)
+    patch[15:18, 40:43] = True
+    psi0[patch & mask] = -1.25
+    _run_pair(params, mask, bits, links, (5, 20, 50), psi0=psi0)

@@ -64,7 +64,7 @@ def _ref_forward(model, x):
     d = c("d2b", c("d2a", torch.cat([up(b), s2], dim=1)))
     d = c("d1b", c("d1a", torch.cat([up(d), s1], dim=1)))
     d = c("d0b", c("d0a", torch.cat([up(d), s0], dim=1)))
-    return c("head", d, relu=False)[:, 0], ws
+    return c("head", d, relu=False)[:, :3].permute(0, 2, 3, 1), ws
 
 
 def test_unet_gradients_match_float64_reference():
@@ -72,7 +72,7 @@ def test_unet_gradients_match_float64_reference():
     torch.manual_seed(0)
     model = UNet(channels=(16, 16, 32, 32), seed=3).cuda()
     x = (torch.randn(2, 16, 32, 8) * 0.5).to(torch.bfloat16).cuda()
-    target = torch.randn(2, 16, 32).cuda()
+    target = torch.randn(2, 16, 32, 3).cuda()
     out = model(x)
     loss = ((out - target) ** 2).mean()
     loss.backward()

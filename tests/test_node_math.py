@@ -149,3 +149,23 @@ def test_zou_he_matches_oracle(node, kw):
             f[4], g[4] = f[2], g[2]
         assert np.allclose(f, o.f[:, y, -1], rtol=0, atol=1e-14), y
         assert np.allclose(g, o.g[:, y, -1], rtol=0, atol=1e-14), y
+
+
+def test_tau_mix_bound(node):
+    """With the |psi| bound the interpolation is the reference's inside
+    [-1, 1] (bit for bit) and its value at the pure phase outside; without
+    it the denominator changes sign at psi = -(1+M)/(M-1)."""
+    node.fdn_inv_tau_mix_f64.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.c_double]
+    node.fdn_inv_tau_mix_f64.restype = ctypes.c_double
+    M, eta = 15.0, 0.1
+    c = np.array([1.0, eta, M, -0.05, 0.1, 1.0, -0.5, 0.0, 0.0])
+    f = lambda rho, psi, b: node.fdn_inv_tau_mix_f64(rho, psi, P(c), b)
+    closed = lambda rho, psi: 1.0 / (6 * M * eta / (rho * ((1 - psi) + M * (1 + psi))) + 0.5)
+    for psi in np.linspace(-1.0, 1.0, 41):
+        assert f(1.02, psi, 1.0) == f(1.02, psi, 0.0)
+        assert abs(f(1.02, psi, 0.0) - closed(1.02, psi)) < 1e-14
+    assert f(1.0, -1.3, 1.0) == f(1.0, -1.0, 0.0)
+    assert f(1.0, 1.2, 1.0) == f(1.0, 1.0, 0.0)
+    sing = -(1 + M) / (M - 1)
+    assert f(1.0, sing - 0.01, 0.0) < 0 < f(1.0, sing + 0.01, 0.0)    # 1/tau flips sign past the pole
