@@ -41,6 +41,20 @@ __device__ __forceinline__ v8bf bf8(u4v v) { return __builtin_bit_cast(v8bf, v);
 
 __device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
+// The bf16 halves of v where the matching halves of m (a ReLU output) are
+// > 0 - for bf16 that is the bit pattern read as a positive int16 - and 0
+// elsewhere: the ReLU's backward mask applied on load.
+__device__ __forceinline__ u4v relu_keep(u4v v, u4v m) {
+    u4v r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const unsigned lo = ((int16_t)(m[i] & 0xFFFFu) > 0) ? 0x0000FFFFu : 0u;
+        const unsigned hi = ((int16_t)(m[i] >> 16) > 0) ? 0xFFFF0000u : 0u;
+        r[i] = v[i] & (lo | hi);
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- forward
 // y[p][co] = act(bias[co] + sum_k A[p][k] w[co][k]), p over N*H*W (NHWC),
 // k = tap*Cin + ci with tap = 3*(dh+1) + (dw+1) and A[p][k] = x at pixel
@@ -50,12 +64,13 @@ __device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast
 // four waves of a workgroup take consecutive pixel ranges with the same
 // channels, so they share the weight fragments through L1.  The K loop
 // takes 32 k per step (one tap and 8 channels per lane, a 16-byte load);
-// k beyond 9*Cin and channels beyond Cout load zeros.
-template <int MP, int NT>
-__global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
-                                                const float* __restrict__ bias, float* __restrict__ y32,
-                                                uint16_t* __restrict__ y16, int N, int H, int W, int Cin, int Cout,
-                                                int relu) {
+// k beyond 9*Cin and channels beyond Cout load zeros.  With MASK, x is
+// multiplied on load by the ReLU mask of xm (the input gradient pass).
+template <int MP, int NT, bool MASK>
+__global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ xm,
+                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                float* __restrict__ y32, uint16_t* __restrict__ y16, int N, int H,
+                                                int W, int Cin, int Cout, int relu) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t P = (int64_t)N * H * W;
     const int64_t pw = ((int64_t)blockIdx.x * 4 + wave) * (16 * MP);
@@ -86,19 +101,29 @@ __global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, 
         const bool kin = k0 < K;
         const int tap = k0 / Cin, ci = k0 - tap * Cin;
         const int dh = tap / 3 - 1, dw = tap % 3 - 1;
-        u4v av[MP], bv[NT];
+        // every load of the step is issued before any is used, so the
+        // waits cover all of them at once
+        u4v av[MP], mv[MP], bv[NT];
 #pragma unroll
         for (int g = 0; g < MP; ++g) {
             av[g] = u4v{0u, 0u, 0u, 0u};
+            mv[g] = u4v{0u, 0u, 0u, 0u};
             const int hh = hq[g] + dh, ww = wq[g] + dw;
-            if (kin && hh >= 0 && hh < H && ww >= 0 && ww < W)
-                av[g] = *reinterpret_cast<const u4v*>(x + ((img[g] + hh) * W + ww) * Cin + ci);
+            if (kin && hh >= 0 && hh < H && ww >= 0 && ww < W) {
+                const int64_t o = ((img[g] + hh) * W + ww) * Cin + ci;
+                av[g] = *reinterpret_cast<const u4v*>(x + o);
+                if (MASK) mv[g] = *reinterpret_cast<const u4v*>(xm + o);
+            }
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int co = co0 + 16 * t + (lane & 15);
             bv[t] = u4v{0u, 0u, 0u, 0u};
             if (co < Cout && kin) bv[t] = *reinterpret_cast<const u4v*>(w + (int64_t)co * K + k0);
+        }
+        if (MASK) {
+#pragma unroll
+            for (int g = 0; g < MP; ++g) av[g] = relu_keep(av[g], mv[g]);
         }
 #pragma unroll
         for (int g = 0; g < MP; ++g)
@@ -147,15 +172,20 @@ __global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__
 // read.  Lane l of wave v stages pixels (l & 31) and (l & 31) + 32 of
 // channel group (l >> 5) + 2v: for a fixed element the 64 lanes write two
 // rows 8 apart (32 banks apart) and 32 consecutive pixels, free of bank
-// conflicts.  A workgroup owns one tap, 64 co x 64 ci and one slice of
+// conflicts.  dy is multiplied on load by the ReLU mask of dym when given.
+// A workgroup owns one tap, 64 co x 64 ci and one slice of
 // pixels; wave v computes co rows 16v..16v+15 (four 16 x 16 tiles, two
-// 32-pixel MFMA steps per stage).  Slices write partial sums that
-// reduce_slices adds in a fixed order.
+// 32-pixel MFMA steps per stage).  The workgroups of tap 0 and the first
+// ci block also form the bias gradient's slice sums, sum_p dy[p][co], with
+// one more MFMA per step against a ones operand (every column of the tile
+// then holds the row sum).  Slices write partial sums that reduce_slices
+// adds in a fixed order.
 constexpr int kStep = 64;
 constexpr int kLds = kStep + 8;
 
 __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
-                                                  float* __restrict__ part, int N, int H, int W, int Cin, int Cout,
+                                                  const uint16_t* __restrict__ dym, float* __restrict__ part,
+                                                  float* __restrict__ bpart, int N, int H, int W, int Cin, int Cout,
                                                   int64_t chunk) {
     __shared__ __align__(16) uint16_t sdy[64][kLds];
     __shared__ __align__(16) uint16_t sx[64][kLds];
@@ -173,6 +203,9 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     v4f acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    v4f accb = {0.f, 0.f, 0.f, 0.f};
+    const bool with_bias = bpart != nullptr && tap == 0 && ib == 0;
+    const u4v ones = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};  // eight bf16 1.0
     const int pl = lane & 31, cg = ((lane >> 5) + 2 * wave) * 8;
     const bool co_in = co0 + cg < Cout, ci_in = ci0 + cg < Cin;
     for (int64_t pc = pbeg; pc < pend; pc += kStep) {
@@ -180,9 +213,12 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
         for (int half = 0; half < 2; ++half) {
             const int pr = pl + 32 * half;
             const int64_t p = pc + pr;
-            u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u};
+            u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u}, vm = {~0u, ~0u, ~0u, ~0u};
             if (p < pend) {
-                if (co_in) vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+                if (co_in) {
+                    vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+                    if (dym) vm = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + cg);
+                }
                 const int wc = (int)(p % W);
                 const int64_t t = p / W;
                 const int h = (int)(t % H), n = (int)(t / H);
@@ -190,6 +226,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
                 if (ci_in && hh >= 0 && hh < H && ww >= 0 && ww < W)
                     vx = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
             }
+            if (dym) vdy = relu_keep(vdy, vm);
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 sdy[cg + e][pr] = (uint16_t)(vdy[e >> 1] >> (16 * (e & 1)));
@@ -206,6 +243,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
                 const v8bf b = bf8(*reinterpret_cast<const u4v*>(&sx[16 * t + (lane & 15)][kp]));
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
             }
+            if (with_bias) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf8(ones), accb, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -219,32 +257,12 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
             if (co < Cout && ci < Cin) part[((int64_t)blockIdx.x * Cout + co) * K + tap * Cin + ci] = acc[t][r];
         }
     }
-}
-
-// per-slice sums of dy over pixels (bias gradient): bpart[s][co].  For
-// each block of up to 256 channels, the workgroup's threads split into
-// 256 / nc interleaved pixel rows; the row sums are added in row order.
-__global__ __launch_bounds__(256) void bias_part(const uint16_t* __restrict__ dy, float* __restrict__ bpart, int64_t P,
-                                                 int Cout, int64_t chunk) {
-    __shared__ float red[256];
-    const int64_t pbeg = (int64_t)blockIdx.x * chunk;
-    const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
-    for (int c0 = 0; c0 < Cout; c0 += 256) {
-        const int nc = std::min(256, Cout - c0);
-        const int rows = 256 / nc;
-        const int co = threadIdx.x % nc, r = threadIdx.x / nc;
-        float s = 0.f;
-        if (r < rows)
-            for (int64_t p = pbeg + r; p < pend; p += rows)
-                s += (float)__builtin_bit_cast(__bf16, dy[p * Cout + c0 + co]);
-        red[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x < nc) {
-            float t = 0.f;
-            for (int q = 0; q < rows; ++q) t += red[q * nc + threadIdx.x];
-            bpart[(int64_t)blockIdx.x * Cout + c0 + threadIdx.x] = t;
+    if (with_bias && (lane & 15) == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
+            if (co < Cout) bpart[(int64_t)blockIdx.x * Cout + co] = accb[r];
         }
-        __syncthreads();
     }
 }
 
@@ -265,9 +283,12 @@ int check(int N, int H, int W, int Cin, int Cout) {
     return FDS_OK;
 }
 
-// pixel slices of the weight gradient: whole 64-pixel steps, at most 256 slices
-void slices(int64_t P, int64_t* chunk, int* S) {
-    int64_t c = (P + 255) / 256;
+// pixel slices of the weight gradient: whole 64-pixel steps, enough slices
+// for about 2048 workgroups in all (1..256), at least 512 pixels each
+void slices(int64_t P, int Cin, int Cout, int64_t* chunk, int* S) {
+    const int64_t per_slice = (int64_t)((Cout + 63) / 64) * 9 * ((Cin + 63) / 64);
+    const int64_t want = std::min<int64_t>(256, std::max<int64_t>(1, (2048 + per_slice - 1) / per_slice));
+    int64_t c = (P + want - 1) / want;
     c = std::max<int64_t>((c + kStep - 1) / kStep * kStep, 512);
     *chunk = c;
     *S = (int)((P + c - 1) / c);
@@ -275,31 +296,51 @@ void slices(int64_t P, int64_t* chunk, int* S) {
 
 }  // namespace
 
-extern "C" {
+namespace {
+template <bool MASK>
+void launch_fwd(const uint16_t* x, const uint16_t* xm, const uint16_t* w, const float* bias, float* y32, uint16_t* y16,
+                int N, int H, int W, int Cin, int Cout, int relu, hipStream_t st) {
+    const int64_t P = (int64_t)N * H * W;
+    // wave tiles: 64 px x 64 co, 128 px x 32 co, or 128 px x 16 co
+    if (Cout >= 64) {
+        const dim3 grid((unsigned)((P + 255) / 256), (unsigned)((Cout + 63) / 64));
+        hipLaunchKernelGGL((conv_fwd<4, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
+                           Cout, relu);
+    } else if (Cout >= 32) {
+        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 31) / 32));
+        hipLaunchKernelGGL((conv_fwd<8, 2, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
+                           Cout, relu);
+    } else {
+        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 15) / 16));
+        hipLaunchKernelGGL((conv_fwd<8, 1, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
+                           Cout, relu);
+    }
+}
 
-const char* fds_last_error(void) { return g_err.c_str(); }
-
-int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* y32, uint16_t* y16, int32_t N,
-                    int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t relu, void* stream) {
+int conv_fwd_checked(const uint16_t* x, const uint16_t* xm, const uint16_t* w, const float* bias, float* y32,
+                     uint16_t* y16, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t relu,
+                     void* stream) {
     int rc = check(N, H, W, Cin, Cout);
     if (rc) return rc;
     if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
     if (!x || !w || (!y32 && !y16)) return fail(FDS_E_ARG, "NULL pointer");
-    const int64_t P = (int64_t)N * H * W;
-    hipStream_t st = (hipStream_t)stream;
-    // wave tiles: 64 px x 64 co, 128 px x 32 co, or 128 px x 16 co
-    if (Cout >= 64) {
-        const dim3 grid((unsigned)((P + 255) / 256), (unsigned)((Cout + 63) / 64));
-        hipLaunchKernelGGL((conv_fwd<4, 4>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
-    } else if (Cout >= 32) {
-        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 31) / 32));
-        hipLaunchKernelGGL((conv_fwd<8, 2>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
-    } else {
-        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 15) / 16));
-        hipLaunchKernelGGL((conv_fwd<8, 1>), grid, dim3(256), 0, st, x, w, bias, y32, y16, N, H, W, Cin, Cout, relu);
-    }
+    if (xm)
+        launch_fwd<true>(x, xm, w, bias, y32, y16, N, H, W, Cin, Cout, relu, (hipStream_t)stream);
+    else
+        launch_fwd<false>(x, nullptr, w, bias, y32, y16, N, H, W, Cin, Cout, relu, (hipStream_t)stream);
     FDS_HIP(hipGetLastError());
     return FDS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* fds_last_error(void) { return g_err.c_str(); }
+
+
+int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* y32, uint16_t* y16, int32_t N,
+                    int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t relu, void* stream) {
+    return conv_fwd_checked(x, nullptr, w, bias, y32, y16, N, H, W, Cin, Cout, relu, stream);
 }
 
 int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout, void* stream) {
@@ -311,22 +352,22 @@ int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout,
     return FDS_OK;
 }
 
-int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, float* dx32, uint16_t* dx16, int32_t N, int32_t H,
-                         int32_t W, int32_t Cin, int32_t Cout, void* stream) {
+int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, const uint16_t* relu_y, float* dx32,
+                         uint16_t* dx16, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream) {
     // the transposed convolution is a forward convolution of dy (Cout
     // channels) with the flipped weights (Cin outputs)
-    return fds_conv3x3_fwd(dy, wt, nullptr, dx32, dx16, N, H, W, Cout, Cin, 0, stream);
+    return conv_fwd_checked(dy, relu_y, wt, nullptr, dx32, dx16, N, H, W, Cout, Cin, 0, stream);
 }
 
 int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
     int64_t chunk;
     int S;
-    slices((int64_t)N * H * W, &chunk, &S);
+    slices((int64_t)N * H * W, Cin, Cout, &chunk, &S);
     return (int64_t)S * Cout * (9 * (int64_t)Cin + 1);
 }
 
-int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, float* dw, float* dbias, float* scratch, int32_t N,
-                           int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream) {
+int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t* relu_y, float* dw, float* dbias,
+                           float* scratch, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream) {
     int rc = check(N, H, W, Cin, Cout);
     if (rc) return rc;
     if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
@@ -334,18 +375,16 @@ int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, float* dw, flo
     const int64_t P = (int64_t)N * H * W;
     int64_t chunk;
     int S;
-    slices(P, &chunk, &S);
+    slices(P, Cin, Cout, &chunk, &S);
     hipStream_t st = (hipStream_t)stream;
     const int64_t nw = (int64_t)Cout * 9 * Cin;
+    float* bp = dbias ? scratch + (int64_t)S * nw : nullptr;
     const dim3 grid((unsigned)S, (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
-    hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, scratch, N, H, W, Cin, Cout, chunk);
+    hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout, chunk);
     FDS_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, scratch, dw, nw, S);
     FDS_HIP(hipGetLastError());
     if (dbias) {
-        float* bp = scratch + (int64_t)S * nw;
-        hipLaunchKernelGGL(bias_part, dim3((unsigned)S), dim3(256), 0, st, dy, bp, P, Cout, chunk);
-        FDS_HIP(hipGetLastError());
         hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((Cout + 255) / 256)), dim3(256), 0, st, bp, dbias,
                            (int64_t)Cout, S);
         FDS_HIP(hipGetLastError());

@@ -18,9 +18,9 @@ _VP, _I32 = ctypes.c_void_p, ctypes.c_int32
 EXPORTS = {
     "fds_conv3x3_fwd": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 6 + [_VP]),
     "fds_flip_weights": (ctypes.c_int, [_VP, _VP, _I32, _I32, _VP]),
-    "fds_conv3x3_bwd_data": (ctypes.c_int, [_VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_conv3x3_bwd_data": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
     "fds_bwd_weight_scratch": (ctypes.c_int64, [_I32] * 5),
-    "fds_conv3x3_bwd_weight": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_conv3x3_bwd_weight": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
     "fds_last_error": (ctypes.c_char_p, []),
 }
 
@@ -85,26 +85,37 @@ def flip_weights(w):
     return wt
 
 
-def conv3x3_bwd_data(dy, wt, out_dtype=torch.bfloat16):
-    """dx of y = conv3x3(x, w), with wt = flip_weights(w)."""
+def _mask(relu_y, dy):
+    if relu_y is not None and (relu_y.shape != dy.shape or relu_y.dtype != torch.bfloat16
+                               or not relu_y.is_contiguous()):
+        raise ValueError("relu_y must be a contiguous bf16 tensor shaped like dy")
+    return relu_y
+
+
+def conv3x3_bwd_data(dy, wt, out_dtype=torch.bfloat16, relu_y=None):
+    """dx of y = conv3x3(x, w), with wt = flip_weights(w); relu_y (the bf16
+    output of a ReLU forward) masks dy on load."""
     N, H, W, Cout = _nhwc(dy, torch.bfloat16)
     Cin = wt.shape[0]
     dx = torch.empty((N, H, W, Cin), dtype=out_dtype, device=dy.device)
-    _check(load().fds_conv3x3_bwd_data(_p(dy), _p(wt), _p(dx if out_dtype == torch.float32 else None),
+    _check(load().fds_conv3x3_bwd_data(_p(dy), _p(wt), _p(_mask(relu_y, dy)),
+                                       _p(dx if out_dtype == torch.float32 else None),
                                        _p(dx if out_dtype == torch.bfloat16 else None), N, H, W, Cin, Cout,
                                        _stream()))
     return dx
 
 
-def conv3x3_bwd_weight(x, dy, with_bias=True, scratch=None):
-    """(dw [Cout,3,3,Cin] float32, dbias [Cout] float32 or None)."""
+def conv3x3_bwd_weight(x, dy, with_bias=True, scratch=None, relu_y=None):
+    """(dw [Cout,3,3,Cin] float32, dbias [Cout] float32 or None); relu_y
+    masks dy on load as in conv3x3_bwd_data."""
     N, H, W, Cin = _nhwc(x, torch.bfloat16)
     Cout = dy.shape[3]
+    _nhwc(dy, torch.bfloat16)
     n = int(load().fds_bwd_weight_scratch(N, H, W, Cin, Cout))
     if scratch is None or scratch.numel() < n:
         scratch = torch.empty(n, dtype=torch.float32, device=x.device)
     dw = torch.empty((Cout, 3, 3, Cin), dtype=torch.float32, device=x.device)
     db = torch.empty(Cout, dtype=torch.float32, device=x.device) if with_bias else None
-    _check(load().fds_conv3x3_bwd_weight(_p(x), _p(dy), _p(dw), _p(db), _p(scratch), N, H, W, Cin, Cout,
-                                         _stream()))
+    _check(load().fds_conv3x3_bwd_weight(_p(x), _p(dy), _p(_mask(relu_y, dy)), _p(dw), _p(db), _p(scratch), N, H,
+                                         W, Cin, Cout, _stream()))
     return dw, db

@@ -37,20 +37,23 @@ int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, flo
  * convolution, so that conv3x3(dy, wt) is the gradient of the input. */
 int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout, void* stream);
 
-/* dx = conv3x3(dy, wt) with wt from fds_flip_weights (dy has Cout
- * channels, dx has Cin).  dx32 and/or dx16 may be NULL. */
-int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, float* dx32, uint16_t* dx16, int32_t N, int32_t H,
-                         int32_t W, int32_t Cin, int32_t Cout, void* stream);
+/* dx = conv3x3(dy', wt) with wt from fds_flip_weights (dy has Cout
+ * channels, dx has Cin).  dy' = dy where relu_y > 0 and 0 elsewhere when
+ * relu_y (the bf16 output of a ReLU forward, same shape as dy) is given,
+ * dy otherwise.  dx32 and/or dx16 may be NULL. */
+int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, const uint16_t* relu_y, float* dx32,
+                         uint16_t* dx16, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream);
 
 /* Number of floats of scratch fds_conv3x3_bwd_weight needs. */
 int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
 
-/* dw[co][tap][ci] = sum over pixels of dy[p][co] * x[p + d(tap)][ci]
- * (float, overwritten); dbias[co] = sum over pixels of dy[p][co] (float,
- * overwritten, may be NULL).  scratch: fds_bwd_weight_scratch floats.
- * Deterministic: fixed-order partial sums, no atomics. */
-int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, float* dw, float* dbias, float* scratch,
-                           int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream);
+/* dw[co][tap][ci] = sum over pixels of dy'[p][co] * x[p + d(tap)][ci]
+ * (float, overwritten); dbias[co] = sum over pixels of dy'[p][co] (float,
+ * overwritten, may be NULL); dy' as for fds_conv3x3_bwd_data (relu_y may
+ * be NULL).  scratch: fds_bwd_weight_scratch floats.  Deterministic:
+ * fixed-order partial sums, no atomics. */
+int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t* relu_y, float* dw, float* dbias,
+                           float* scratch, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream);
 
 /* Message of the last failed call on this thread. */
 const char* fds_last_error(void);

@@ -79,3 +79,22 @@ def test_flip_weights_is_an_involution_with_transpose():
     ref = w.cpu().flip(1, 2).permute(3, 1, 2, 0).contiguous()
     assert torch.equal(wt.cpu(), ref)
     assert torch.equal(so.flip_weights(wt).cpu(), w.cpu())
+
+
+def test_relu_mask_on_load_equals_premasked_gradient():
+    """relu_y masks dy inside the kernels: bit-identical to masking first."""
+    from fingering_dynamics_amd import surrogate_ops as so
+    g = torch.Generator().manual_seed(11)
+    N, H, W, Cin, Cout = 2, 12, 20, 16, 32
+    x = _bf(torch.randn(N, H, W, Cin, generator=g)).cuda()
+    w = _bf(torch.randn(Cout, 3, 3, Cin, generator=g) * 0.2).cuda()
+    y = so.conv3x3(x, w, torch.zeros(Cout).cuda(), relu=True)     # bf16, zeros where masked
+    y[0, 0, 0, :4] = -0.0                                          # negative zero counts as masked
+    dy = _bf(torch.randn(N, H, W, Cout, generator=g)).cuda()
+    pre = (dy * (y > 0)).contiguous()
+    wt = so.flip_weights(w)
+    assert torch.equal(so.conv3x3_bwd_data(dy, wt, relu_y=y), so.conv3x3_bwd_data(pre, wt))
+    dw_a, db_a = so.conv3x3_bwd_weight(x, dy, relu_y=y)
+    dw_b, db_b = so.conv3x3_bwd_weight(x, pre)
+    assert torch.equal(dw_a, dw_b) and torch.equal(db_a, db_b)
+    assert float((y > 0).float().mean()) < 0.9                     # the mask removes something
