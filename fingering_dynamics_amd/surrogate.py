@@ -43,6 +43,8 @@ from . import output, surrogate_ops as so
 
 U_REF = 1e-3          # velocity scale of the inputs (the workload's inlet velocity)
 LAMBDA_U = 0.01       # weight of the velocity change in the training loss
+LAMBDA_S = 0.0        # weight of the phase (sign) cross-entropy on interface cells (0.01 tried)
+SIGN_GAIN = 4.0       # logit = SIGN_GAIN * predicted psi
 CHANNELS = (32, 64, 128, 128)
 IN_CH = 8
 
@@ -323,7 +325,8 @@ def generate(n_domains=64, H=256, W=512, f=8, snap=500, max_steps=200000, seed=4
             "steps": steps, "specs": [specs[i] for i in ok], "dropped": n_domains - len(ok), "f": f,
             "valid": [valid[i] for i in ok], "non_finite": int(sum(not fin_h[i].all() for i in range(n_domains))),
             "broke_through": int(bt_h.any(axis=1).sum()),
-            "seconds": gen_s, "lattice_updates": n_domains * H * W * max_steps}
+            "seconds": gen_s, "lattice_updates": n_domains * H * W * max_steps,
+            "flows": [(cs[i].mask, cs[i].links, cs[i].params) for i in ok]}
     ens.close()
     for c in cs:
         c.close()
@@ -361,19 +364,35 @@ def batch(data, items, d, flip=None):
 
 
 def loss_fn(model, x, psi0, psi1, v0, v1, wfluid):
-    """Fluid-weighted MSE of psi(t + K) plus LAMBDA_U times that of the
-    velocity / U_REF."""
+    """Fluid-weighted MSE of psi(t + K), plus LAMBDA_U times that of the
+    velocity / U_REF, plus LAMBDA_S times the cross-entropy of the phase
+    (psi > 0) at t + K with logit SIGN_GAIN * predicted psi, over cells
+    at the interface at t or t + K (|psi| < 0.9).  The MSE alone smears an
+    uncertain finger-tip advance into small increments over several cells;
+    the phase term rewards committing to the cells that change phase (it
+    makes the one-step phase prediction decisive, but did not shorten the
+    coupled run's breakthrough, so it is off by default)."""
     out = model(x)
     w = wfluid.sum().clamp(min=1.0)
-    lp = ((psi0 + out[..., 0] - psi1) ** 2 * wfluid).sum() / w
+    pred = psi0 + out[..., 0]
+    lp = ((pred - psi1) ** 2 * wfluid).sum() / w
     lu = (((v0 + out[..., 1:3] - v1) ** 2).sum(dim=-1) * wfluid).sum() / w
-    return lp + LAMBDA_U * lu
+    band = ((psi0.abs() < 0.9) | (psi1.abs() < 0.9)).float() * wfluid
+    if LAMBDA_S == 0.0:
+        return lp + LAMBDA_U * lu
+    ls = torch.nn.functional.binary_cross_entropy_with_logits(
+        SIGN_GAIN * pred, (psi1 > 0).float(), weight=band, reduction="sum") / band.sum().clamp(min=1.0)
+    return lp + LAMBDA_U * lu + LAMBDA_S * ls
 
 
-def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-4, seed=0, log=print):
+def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-4, seed=0, extra=None,
+          extra_frac=0.5, log=print):
     """AdamW on the FP32 master weights with a cosine schedule; random
-    (domain, t) pairs, half of them mirrored.  Returns (loss history, s)."""
+    (domain, t) pairs, half of them mirrored.  With `extra` (coupled_pairs),
+    a fraction extra_frac of every batch comes from those pairs instead.
+    Returns (loss history, s)."""
     items, d = pairs(data, train_idx, jump, snap)
+    n_extra = int(round(bs * extra_frac)) if extra is not None else 0
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device="cuda").manual_seed(seed)
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=wd)
@@ -381,9 +400,14 @@ def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-
     hist = []
     t0 = time.perf_counter()
     for it in range(iters):
-        sel = [items[j] for j in rng.integers(0, len(items), bs)]
+        sel = [items[j] for j in rng.integers(0, len(items), bs - n_extra)]
         flip = torch.rand(bs, device="cuda", generator=gen) < 0.5
-        loss = loss_fn(model, *batch(data, sel, d, flip))
+        b = batch(data, sel, d, flip[:bs - n_extra])
+        if n_extra:
+            e = batch_pairs(extra, torch.as_tensor(rng.integers(0, extra["n"], n_extra), device="cuda"),
+                            flip[bs - n_extra:])
+            b = [torch.cat([u, v]) for u, v in zip(b, e)]
+        loss = loss_fn(model, *b)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
@@ -403,6 +427,7 @@ def evaluate(model, data, idx, jump, snap, bs=64):
     domains idx."""
     items, d = pairs(data, idx, jump, snap)
     acc = {"mse": 0.0, "mse_persist": 0.0, "ierr": 0.0, "ierr_persist": 0.0, "mse_u": 0.0, "mse_u_persist": 0.0}
+    flips_pred = flips_true = flips_hit = 0.0
     n = 0
     for k in range(0, len(items), bs):
         sel = items[k:k + bs]
@@ -414,13 +439,141 @@ def evaluate(model, data, idx, jump, snap, bs=64):
         acc["mse_persist"] += float(((p0 - p1) ** 2 * wf).sum() / w) * len(sel)
         acc["mse_u"] += float((((v0 + out[..., 1:3] - v1) ** 2).sum(dim=-1) * wf).sum() / w) * len(sel)
         acc["mse_u_persist"] += float((((v0 - v1) ** 2).sum(dim=-1) * wf).sum() / w) * len(sel)
+        m = (wf > 0.5) & (p0 <= 0)
+        tp, pp = (p1 > 0) & m, (pred > 0) & m
+        flips_true += float(tp.sum())
+        flips_pred += float(pp.sum())
+        flips_hit += float((tp & pp).sum())
         acc["ierr"] += float(interface_error(pred, p1, wf)) * len(sel)
         acc["ierr_persist"] += float(interface_error(p0, p1, wf)) * len(sel)
         n += len(sel)
-    return {k: v / max(n, 1) for k, v in acc.items()}
+    res = {k: v / max(n, 1) for k, v in acc.items()}
+    # cells invaded during the jump: predicted / true count, and the share
+    # of the true ones predicted
+    res["flip_ratio"] = flips_pred / max(flips_true, 1.0)
+    res["flip_recall"] = flips_hit / max(flips_true, 1.0)
+    return res
+
+
+# ------------------------------------------------ pairs on coupled states
+@torch.no_grad()
+def coupled_pairs(model, flows, specs, f, jump=2000, relax=200, cycles=100, band=2, log=print):
+    """Training pairs on the states the accelerated loop produces (the
+    learner's own state distribution, labelled by the solver): the flows
+    run the coupled loop together (one batched launch per iteration); at
+    every cycle each flow's coarse state is the input, and the target is
+    the coarse state after `jump` plain iterations from that same state
+    (run, recorded, then rolled back through the device state blobs)
+    before the jump is applied.  Pairs from flows that have broken
+    through or gone non-finite are left out.  Returns a dict of device
+    tensors psi0, ux0, uy0, psi1, ux1, uy1 [N][h][w], the flow index of
+    each pair, solid [D][h][w], M, theta [D] and n."""
+    from .ensemble import Ensemble
+    from .lbm import Compute
+    cs = [Compute(m, l, p) for m, l, p in flows]
+    ens = Ensemble(cs)
+    ens.set_tile(1, 16)
+    D = len(cs)
+    H, W = cs[0].H, cs[0].W
+    col = probe_column(W)
+    fl = [torch.as_tensor(c.mask, dtype=torch.float32).cuda() for c in cs]
+    n_c = torch.stack([v.view(H // f, f, W // f, f).sum(dim=(1, 3)) for v in fl])
+    fluid_c = (n_c > 0).float()
+    solid = 1.0 - n_c / (f * f)
+    inflow = [2.0 * c.params.u0 * float(v[:, 0].sum()) * jump for c, v in zip(cs, fl)]
+    M = torch.tensor([sp["M"] for sp in specs], dtype=torch.float32, device="cuda")
+    th = torch.tensor([180.0 - sp["theta"] for sp in specs], dtype=torch.float32, device="cuda")
+    bufs, tbufs, blobs = [None] * D, [None] * D, [None] * D
+    X, Y, who = [], [], []
+    alive = np.ones(D, dtype=bool)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(relax)
+    for k in range(cycles):
+        xs = []
+        for i, c in enumerate(cs):
+            with torch.cuda.stream(c.torch_stream()):
+                bufs[i] = c.fields_device(bufs[i])
+                xs.append(coarsen(bufs[i][[0, 2, 3]], fl[i], f))
+                blobs[i] = c.save_state_device(blobs[i][0] if blobs[i] else None)
+        t = cs[0].t
+        ens.step(jump)
+        ys, ok = [], []
+        for i, c in enumerate(cs):
+            with torch.cuda.stream(c.torch_stream()):
+                tbufs[i] = c.fields_device(tbufs[i])
+                ys.append(coarsen(tbufs[i][[0, 2, 3]], fl[i], f))
+                hit, bad = status(bufs[i], fl[i], col)
+                ok.append(not hit and not bad and bool(torch.isfinite(ys[-1]).all()))
+                c.load_state_device((blobs[i][0], t))
+        torch.cuda.synchronize()
+        alive &= np.array(ok)
+        xs, ys = torch.stack(xs), torch.stack(ys)
+        for i in np.nonzero(alive)[0]:
+            X.append(xs[i])
+            Y.append(ys[i])
+            who.append(int(i))
+        # the jump itself, as accelerated_run applies it (velocity kept)
+        out = model(inputs(xs[:, 0], xs[:, 1], xs[:, 2], solid, M, th))
+        torch.cuda.synchronize()    # the handles' streams read `out` next
+        for i, c in enumerate(cs):
+            with torch.cuda.stream(c.torch_stream()):
+                delta, _ = jump_change(xs[i, 0], out[i, ..., 0], fluid_c[i], n_c[i], band, inflow[i])
+                b = bufs[i]
+                psi_new = torch.where(fl[i] > 0, (b[0] + refine(delta, f)).clamp(-1.0, 1.0), b[0])
+                c.init_fields_device(psi_new, b[1], b[2], b[3])
+        torch.cuda.synchronize()
+        ens.step(relax)
+        if k % 10 == 0:
+            log("coupled pairs: cycle %d, %d flows alive (%.1f s)" % (k, int(alive.sum()), time.perf_counter() - t0))
+        if not alive.any():
+            break
+    torch.cuda.synchronize()
+    ens.close()
+    for c in cs:
+        c.close()
+    X, Y = torch.stack(X), torch.stack(Y)
+    return {"psi0": X[:, 0].contiguous(), "ux0": X[:, 1].contiguous(), "uy0": X[:, 2].contiguous(),
+            "psi1": Y[:, 0].contiguous(), "ux1": Y[:, 1].contiguous(), "uy1": Y[:, 2].contiguous(),
+            "flow": torch.tensor(who, device="cuda"), "solid": solid, "M": M, "theta": 180.0 - th,
+            "n": len(who), "seconds": time.perf_counter() - t0}
+
+
+def batch_pairs(cp, idx, flip=None):
+    """batch() for coupled_pairs: pairs idx (device tensor)."""
+    fi = cp["flow"][idx]
+    p0, p1, ux0, uy0, ux1, uy1 = [cp[k][idx] for k in ("psi0", "psi1", "ux0", "uy0", "ux1", "uy1")]
+    sol = cp["solid"][fi]
+    if flip is not None:
+        fl = flip.view(-1, 1, 1)
+        p0, p1, ux0, ux1, sol = [torch.where(fl, v.flip(-2), v) for v in (p0, p1, ux0, ux1, sol)]
+        uy0, uy1 = [torch.where(fl, -v.flip(-2), v) for v in (uy0, uy1)]
+    x = inputs(p0, ux0, uy0, sol, cp["M"][fi], 180.0 - cp["theta"][fi])
+    v0 = torch.stack([ux0, uy0], dim=-1) / U_REF
+    v1 = torch.stack([ux1, uy1], dim=-1) / U_REF
+    return x, p0, p1, v0, v1, 1.0 - sol
 
 
 # ------------------------------------------------------- plain and accelerated runs
+def jump_change(pc, dpsi, fluid_c, n_c, band, inflow=None):
+    """The coarse psi change a jump applies: the predicted change dpsi
+    (with the prediction clamped to [-1, 1]) on fluid cells within `band`
+    cells of the interface (|pc| < 0.9; 0: everywhere), scaled when inflow
+    is given so that sum(change * n_c) = inflow (clamped to [0.5, 2]).
+    Returns (change, scale)."""
+    delta = ((pc + dpsi).clamp(-1.0, 1.0) - pc) * fluid_c
+    if band > 0:
+        near = (pc.abs() < 0.9).float()[None, None]
+        near = torch.nn.functional.max_pool2d(near, 2 * band + 1, stride=1, padding=band)[0, 0]
+        delta = delta * near
+    scale = torch.ones((), device=pc.device)
+    if inflow is not None:
+        gain = (delta * n_c).sum()
+        scale = torch.where(gain > 0, inflow / gain.clamp(min=1e-6), scale).clamp(0.5, 2.0)
+        delta = delta * scale
+    return delta, scale
+
+
 @torch.no_grad()
 def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
     """Plain LBM from cm's current (fresh) state until breakthrough at the
@@ -511,16 +664,8 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
             pc, uxc, uyc = coarsen(buf[[0, 2, 3]], fluid, f)
             x = inputs(pc[None], uxc[None], uyc[None], solid_c, Mt, Tt)
             out = model(x)[0]
-            pred = (pc + out[..., 0]).clamp(-1.0, 1.0)
-            delta = (pred - pc) * fluid_c
-            if band > 0:
-                near = (pc.abs() < 0.9).float()[None, None]
-                near = torch.nn.functional.max_pool2d(near, 2 * band + 1, stride=1, padding=band)[0, 0]
-                delta = delta * near
+            delta, scale = jump_change(pc, out[..., 0], fluid_c, n_c, band, inflow if mass_fix else None)
             if mass_fix:
-                gain = (delta * n_c).sum()
-                scale = torch.where(gain > 0, inflow / gain.clamp(min=1e-6), torch.ones_like(gain)).clamp(0.5, 2.0)
-                delta = delta * scale
                 stats["mass_scale"].append(scale)
             psi_new = torch.where(fluid > 0, (buf[0] + refine(delta, f)).clamp(-1.0, 1.0), buf[0])
             if velocity:

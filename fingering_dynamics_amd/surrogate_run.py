@@ -95,10 +95,17 @@ def main(argv=None):
     ap.add_argument("--variants", default="",
                     help="further accelerated runs as relax:velocity-scale:band, e.g. 500:0:2,1000:0.5:2")
     ap.add_argument("--band", type=int, default=2, help="coarse cells around the interface a jump may change")
+    ap.add_argument("--dagger-cycles", type=int, default=0,
+                    help="coupled-loop cycles on the training flows whose states, labelled by the solver, "
+                         "fine-tune the network (0: none)")
+    ap.add_argument("--dagger-iters", type=int, default=4000)
+    ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
+                    help="weight of the phase cross-entropy term of the training loss")
     ap.add_argument("--regime", default="target", choices=sorted(sg.REGIMES))
     ap.add_argument("--velocity", type=float, default=0.0,
                     help="scale of the predicted velocity change applied at each jump (0: keep the solver's)")
     a = ap.parse_args(argv)
+    sg.LAMBDA_S = a.phase_weight
     os.makedirs(a.out, exist_ok=True)
     res = {"args": vars(a)}
     model = sg.UNet(seed=0).cuda()
@@ -128,6 +135,16 @@ def main(argv=None):
         res["eval_train"] = sg.evaluate(model, data, tr[:hold], a.jump, a.snap)
         res["eval_heldout"] = sg.evaluate(model, data, te, a.jump, a.snap)
         log("held-out: %s" % json.dumps(res["eval_heldout"]))
+        if a.dagger_cycles > 0:
+            cp = sg.coupled_pairs(model, [data["flows"][i] for i in tr], [data["specs"][i] for i in tr], a.f,
+                                  a.jump, a.relax, a.dagger_cycles, a.band, log=log)
+            hist2, secs2 = sg.train(model, data, tr, a.jump, a.snap, iters=a.dagger_iters, bs=a.bs, lr=a.lr * 0.5,
+                                    seed=1, extra=cp, log=log)
+            res["dagger"] = {"pairs": cp["n"], "seconds": cp["seconds"], "train_seconds": secs2,
+                             "loss_last": hist2[-1][1]}
+            res["eval_heldout_after_dagger"] = sg.evaluate(model, data, te, a.jump, a.snap)
+            log("held-out after coupled pairs: %s" % json.dumps(res["eval_heldout_after_dagger"]))
+            del cp
         save_model(model, os.path.join(a.out, "unet.npz"))
         del data
         torch.cuda.empty_cache()

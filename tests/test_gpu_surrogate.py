@@ -204,3 +204,31 @@ def test_forced_fallback_is_the_plain_lbm():
     assert c.mass()[2] == 0
     for s_ in (a, b, c):
         s_.close()
+
+
+def test_coupled_pairs_start_from_the_plain_state():
+    """coupled_pairs: one pair per live flow and cycle; the first input is
+    the plain LBM's coarse state after `relax` iterations, bit for bit, and
+    the first target that after relax + jump."""
+    from fingering_dynamics_amd import surrogate as sg
+    from fingering_dynamics_amd.lbm import Compute
+    specs = [sg.flow_spec(k, H=64, W=256, regime="target") for k in range(2)]
+    flows = [sg.build_flow(s) for s in specs]
+    model = sg.UNet(seed=2).cuda().eval()
+    cp = sg.coupled_pairs(model, flows, specs, 8, jump=400, relax=100, cycles=3, band=2, log=print)
+    assert cp["n"] == 6 and cp["psi0"].shape == (6, 8, 32)
+    assert sorted(cp["flow"].tolist()) == [0, 0, 0, 1, 1, 1]
+    for v in ("psi0", "psi1", "ux0", "ux1"):
+        assert bool(torch.isfinite(cp[v]).all())
+    m, l, p = flows[1]
+    c = Compute(m, l, p)
+    fl = torch.as_tensor(m, dtype=torch.float32).cuda()
+    c.step(100)
+    first = sg.coarsen(torch.as_tensor(c.fields()[0]).cuda(), fl, 8)
+    c.step(400)
+    teach = sg.coarsen(torch.as_tensor(c.fields()[0]).cuda(), fl, 8)
+    k = cp["flow"].tolist().index(1)
+    assert torch.equal(cp["psi0"][k], first)
+    assert torch.equal(cp["psi1"][k], teach)
+    assert not torch.equal(cp["psi0"][k], cp["psi1"][k])
+    c.close()
