@@ -99,6 +99,8 @@ def main(argv=None):
                     help="coupled-loop cycles on the training flows whose states, labelled by the solver, "
                          "fine-tune the network (0: none)")
     ap.add_argument("--dagger-iters", type=int, default=4000)
+    ap.add_argument("--workload-seed", type=int, default=None,
+                    help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
                     help="weight of the phase cross-entropy term of the training loss")
     ap.add_argument("--regime", default="target", choices=sorted(sg.REGIMES))
@@ -152,6 +154,8 @@ def main(argv=None):
     log("mfma: %s" % json.dumps(res["mfma"]))
 
     spec = sg.breakthrough_spec()
+    if a.workload_seed is not None:
+        spec["obstacle_seed"] = a.workload_seed
     mask, links, params = sg.build_flow(spec)
     res["workload"] = dict(spec, porosity_reached=float(mask.mean()), psi_wall=params.psi_wall, u0=params.u0,
                            probe_column=sg.probe_column(spec["W"]))
