@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1333,10 +1334,35 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
 
 void* fd_stream(const fd_handle* h) { return h ? (void*)h->stream : nullptr; }
 
+}  // extern "C"
+
+namespace {
+// 0 if every non-NULL pointer is device memory of the handle's device, else
+// an FD_E_ARG failure: a host pointer in a device-to-device copy would
+// fault the GPU instead of failing the call.
+int check_device_ptrs(const fd_handle* h, std::initializer_list<const void*> ptrs) {
+    for (const void* q : ptrs) {
+        if (!q) continue;
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FD_E_ARG, "not a device pointer");
+        }
+        if (a.type != hipMemoryTypeDevice || a.device != h->device)
+            return fail(FD_E_ARG, "not device memory of the handle's device");
+    }
+    return FD_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int fd_fields_device(fd_handle* h, float* psi, float* rho, float* ux, float* uy) {
     if (!h) return fail(FD_E_ARG, "handle is NULL");
     FD_HIP(hipSetDevice(h->device));
-    int rc = compute_fields(h, nullptr);
+    int rc = check_device_ptrs(h, {psi, rho, ux, uy});
+    if (rc) return rc;
+    rc = compute_fields(h, nullptr);
     if (rc) return rc;
     float* outs[4] = {psi, rho, ux, uy};
     for (int i = 0; i < 4; ++i)
@@ -1350,6 +1376,8 @@ int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, cons
     if (!h || !psi || !rho || !ux || !uy) return fail(FD_E_ARG, "handle or a field is NULL");
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
     FD_HIP(hipSetDevice(h->device));
+    int rc = check_device_ptrs(h, {psi, rho, ux, uy});
+    if (rc) return rc;
     // Padding columns x >= W of the staging planes are never read by the
     // initialisation (it covers x < W only).
     const float* src[4] = {psi, rho, ux, uy};
@@ -1384,6 +1412,8 @@ int fd_save_state_device(fd_handle* h, void* dbuf, int64_t nbytes) {
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
     if (nbytes < fd_state_device_bytes(h)) return fail(FD_E_ARG, "buffer too small");
     FD_HIP(hipSetDevice(h->device));
+    int rc = check_device_ptrs(h, {dbuf});
+    if (rc) return rc;
     char* p = (char*)dbuf;
     FD_HIP(hipMemcpyAsync(p, h->side[h->sc], sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToDevice, h->stream));
     FD_HIP(hipMemcpyAsync(p + sizeof(float) * 4 * h->p.H, h->pop[h->cur], sizeof(float) * 18 * h->Q,
@@ -1396,6 +1426,8 @@ int fd_load_state_device(fd_handle* h, const void* dbuf, int64_t nbytes, int64_t
     if (nbytes < fd_state_device_bytes(h)) return fail(FD_E_ARG, "buffer too small");
     if (steps_done < 1) return fail(FD_E_ARG, "a device state is restored at iteration >= 1");
     FD_HIP(hipSetDevice(h->device));
+    int rc = check_device_ptrs(h, {dbuf});
+    if (rc) return rc;
     const char* p = (const char*)dbuf;
     FD_HIP(hipMemcpyAsync(h->side[h->sc], p, sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToDevice, h->stream));
     FD_HIP(hipMemcpyAsync(h->pop[h->cur], p + sizeof(float) * 4 * h->p.H, sizeof(float) * 18 * h->Q,

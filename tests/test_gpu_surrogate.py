@@ -160,6 +160,11 @@ def test_device_coupling_matches_host_path():
         assert np.array_equal(x, y)
     with pytest.raises(ValueError):
         a.init_fields_device(t[0].double(), *t[1:])
+    # host memory handed to a device entry point is an argument error, not a fault
+    host = np.zeros((p.H, p.W), dtype=np.float32)
+    from fingering_dynamics_amd import _native
+    assert a._lib.fd_fields_device(a._h, _native.ptr(host), None, None, None) == -1
+    assert a._lib.fd_init_fields_device(a._h, *[_native.ptr(host)] * 4) == -1
     for s_ in (a, b, c):
         s_.close()
 
