@@ -609,7 +609,7 @@ def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
 
 @torch.no_grad()
 def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=0.02, max_steps=10 ** 7,
-                    mass_fix=True, velocity=0.0, band=2, log=print):
+                    mass_fix=True, velocity=0.0, band=2, teacher=None, log=print):
     """Surrogate-accelerated run of solver cm (freshly initialised) until
     breakthrough, every array on the device and every launch on the
     solver's stream.
@@ -630,6 +630,9 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
     prediction and the relaxed coarse psi is judged, and above `threshold`
     the state before the jump is restored and the plain LBM runs K + relax
     iterations instead.  Breakthrough is checked after every cycle.
+    With `teacher` ({iteration: coarse psi} of a plain run) the jump's
+    target is the plain run's coarse psi K iterations later instead of the
+    network's (a diagnostic of the restart with a perfect coarse predictor).
     Returns (breakthrough model time or None, wall seconds, stats,
     {model time: coarse psi})."""
     H, W = cm.H, cm.W
@@ -664,6 +667,9 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
             pc, uxc, uyc = coarsen(buf[[0, 2, 3]], fluid, f)
             x = inputs(pc[None], uxc[None], uyc[None], solid_c, Mt, Tt)
             out = model(x)[0]
+            if teacher is not None and t_model + jump in teacher:
+                out = out.clone()
+                out[..., 0] = teacher[t_model + jump] - pc
             delta, scale = jump_change(pc, out[..., 0], fluid_c, n_c, band, inflow if mass_fix else None)
             if mass_fix:
                 stats["mass_scale"].append(scale)

@@ -99,6 +99,8 @@ def main(argv=None):
                     help="coupled-loop cycles on the training flows whose states, labelled by the solver, "
                          "fine-tune the network (0: none)")
     ap.add_argument("--dagger-iters", type=int, default=4000)
+    ap.add_argument("--teacher", action="store_true",
+                    help="diagnostic: jumps to the plain run's own coarse psi (perfect coarse predictor)")
     ap.add_argument("--workload-seed", type=int, default=None,
                     help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
@@ -163,7 +165,8 @@ def main(argv=None):
     rec_p = {}
     if not a.skip_plain:
         cm = Compute(mask, links, params)
-        t_b, wall, rec_p = sg.plain_run(cm, a.f, a.max_steps, chunk=a.relax, record_every=cycle, log=log)
+        t_b, wall, rec_p = sg.plain_run(cm, a.f, a.max_steps, chunk=a.relax,
+                                        record_every=a.relax if a.teacher else cycle, log=log)
         res["plain"] = {"breakthrough_t": t_b, "wall_s": wall, "mlups": cm.t * mask.size / wall / 1e6,
                         "non_finite": cm.mass()[2]}
         cm.close()
@@ -182,7 +185,8 @@ def main(argv=None):
         cm = Compute(mask, links, params)
         t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, model, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
                                                        relax, a.threshold, a.max_steps,
-                                                       mass_fix=not a.no_mass_fix, velocity=vel, band=band, log=log)
+                                                       mass_fix=not a.no_mass_fix, velocity=vel, band=band,
+                                                       teacher=rec_p if a.teacher else None, log=log)
         acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
                    non_finite=cm.mass()[2])
         cm.close()
