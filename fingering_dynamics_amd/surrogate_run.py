@@ -16,14 +16,17 @@
 
 Writes DIR/surrogate_run.json (every figure above) and DIR/unet.npz (the
 trained weights); --load-model NPZ skips steps 1-3.  `--mfma-only` runs
-step 4 alone (for profiling the convolution kernels).
+step 4 alone (for profiling the convolution kernels).  Further options:
+--variants (more accelerated runs: relax, velocity scale, band),
+--dagger-cycles (fine-tuning on the coupled loop's own states, labelled
+by the solver), --phase-weight (the phase cross-entropy term of the loss),
+--teacher (jumps to the plain run's own coarse future: the coupling with
+a perfect coarse predictor), --workload-seed (another obstacle bed).
 """
 import argparse
 import json
-import math
 import os
 import sys
-import time
 
 import numpy as np
 import torch
