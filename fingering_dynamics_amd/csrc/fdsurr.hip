@@ -266,14 +266,147 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     }
 }
 
-// out[i] = sum over s of part[s][i], s in order
+// Weight gradient of thin layers (Cout <= 32): one workgroup owns a block
+// of up to 32 input channels, all output channels and all 9 taps, and a
+// slice of row segments (64 consecutive pixels of one image row).  Per
+// segment it stages dy (transposed, [co][pixel]) and the three x rows the
+// taps reach, each in three copies shifted by dw = -1, 0, +1
+// ([dh][dw][ci][pixel]), so every tap's fragments are aligned 16-byte LDS
+// reads and x is read once for all taps.  Wave v computes taps v, v+4, v+8
+// (< 9), 16 x 16 tiles skipped where the channels end; wave 3 of the first
+// channel block also forms the bias sums (ones operand).  Partial sums per
+// slice, added by reduce_slices.
+constexpr int kSmallCo = 32, kSmallCi = 32;
+
+__global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restrict__ x,
+                                                        const uint16_t* __restrict__ dy,
+                                                        const uint16_t* __restrict__ dym, float* __restrict__ part,
+                                                        float* __restrict__ bpart, int N, int H, int W, int Cin,
+                                                        int Cout, int64_t segs_per_slice) {
+    __shared__ __align__(16) uint16_t sdy[kSmallCo][kLds];
+    __shared__ __align__(16) uint16_t sx[9][kSmallCi][kLds];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ci0 = blockIdx.y * kSmallCi;
+    const int wsegs = (W + 63) / 64;
+    const int64_t nseg = (int64_t)N * H * wsegs;
+    const int64_t sbeg = (int64_t)blockIdx.x * segs_per_slice;
+    const int64_t send = std::min<int64_t>(nseg, sbeg + segs_per_slice);
+    const bool co2 = Cout > 16, ci2 = ci0 + 16 < Cin;
+    const bool with_bias = bpart != nullptr && blockIdx.y == 0 && wave == 3;
+    const u4v ones = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};
+    v4f acc[3][2][2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[j][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    v4f accb[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+    for (int64_t sg = sbeg; sg < send; ++sg) {
+        const int w0 = (int)(sg % wsegs) * 64;
+        const int64_t nh = sg / wsegs;
+        const int h = (int)(nh % H);
+        const int64_t n = nh / H;
+        // dy: 64 pixels x 4 groups of 8 output channels, one piece per thread
+        {
+            const int pp = threadIdx.x >> 2, g = threadIdx.x & 3;
+            u4v v = {0u, 0u, 0u, 0u}, m = {~0u, ~0u, ~0u, ~0u};
+            if (w0 + pp < W && 8 * g < Cout) {
+                const int64_t o = ((n * H + h) * W + w0 + pp) * Cout + 8 * g;
+                v = *reinterpret_cast<const u4v*>(dy + o);
+                if (dym) m = *reinterpret_cast<const u4v*>(dym + o);
+            }
+            if (dym) v = relu_keep(v, m);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sdy[8 * g + e][pp] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+        }
+        // x: 3 rows x 66 columns (w0-1 .. w0+64) x 4 groups of 8 input
+        // channels; column c lands at pixel c - dw of copy dw
+        for (int i = threadIdx.x; i < 3 * 66 * 4; i += 256) {
+            const int g = i & 3, c = (i >> 2) % 66 - 1, r = (i >> 2) / 66;
+            const int hh = h + r - 1, ww = w0 + c;
+            u4v v = {0u, 0u, 0u, 0u};
+            if (hh >= 0 && hh < H && ww >= 0 && ww < W && ci0 + 8 * g < Cin)
+                v = *reinterpret_cast<const u4v*>(x + ((n * H + hh) * W + ww) * Cin + ci0 + 8 * g);
+#pragma unroll
+            for (int dw = -1; dw <= 1; ++dw) {
+                const int pp = c - dw;
+                if (pp < 0 || pp >= 64) continue;
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    sx[3 * r + dw + 1][8 * g + e][pp] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            const int kp = 32 * kk + 8 * (lane >> 4);
+            v8bf a[2];
+            a[0] = bf8(*reinterpret_cast<const u4v*>(&sdy[lane & 15][kp]));
+            a[1] = bf8(*reinterpret_cast<const u4v*>(&sdy[16 + (lane & 15)][kp]));
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int tap = wave + 4 * j;
+                if (tap >= 9) continue;
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    if (b == 1 && !ci2) continue;
+                    const v8bf bv = bf8(*reinterpret_cast<const u4v*>(&sx[tap][16 * b + (lane & 15)][kp]));
+                    acc[j][0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bv, acc[j][0][b], 0, 0, 0);
+                    if (co2) acc[j][1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bv, acc[j][1][b], 0, 0, 0);
+                }
+            }
+            if (with_bias) {
+                accb[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bf8(ones), accb[0], 0, 0, 0);
+                if (co2) accb[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bf8(ones), accb[1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const int64_t K = 9 * (int64_t)Cin;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int tap = wave + 4 * j;
+        if (tap >= 9) continue;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int ci = ci0 + 16 * b + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int co = 16 * a + 4 * (lane >> 4) + r;
+                    if (co < Cout && ci < Cin && ci < ci0 + kSmallCi)
+                        part[((int64_t)blockIdx.x * Cout + co) * K + tap * Cin + ci] = acc[j][a][b][r];
+                }
+            }
+    }
+    if (with_bias && (lane & 15) == 0) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = 16 * a + 4 * (lane >> 4) + r;
+                if (co < Cout) bpart[(int64_t)blockIdx.x * Cout + co] = accb[a][r];
+            }
+    }
+}
+
+// out[i] = sum over s of part[s][i]: eight interleaved partial sums (slices
+// s = j mod 8) keep eight loads in flight per thread, then a fixed tree;
+// the order is the same on every call.
 __global__ __launch_bounds__(256) void reduce_slices(const float* __restrict__ part, float* __restrict__ out,
                                                      int64_t n, int S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0.f;
-    for (int k = 0; k < S; ++k) s += part[(int64_t)k * n + i];
-    out[i] = s;
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int k = 0;
+    for (; k + 8 <= S; k += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] += part[(int64_t)(k + j) * n + i];
+    }
+    for (int j = 0; k < S; ++k, ++j) a[j] += part[(int64_t)k * n + i];
+    out[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
 int check(int N, int H, int W, int Cin, int Cout) {
@@ -283,15 +416,37 @@ int check(int N, int H, int W, int Cin, int Cout) {
     return FDS_OK;
 }
 
-// pixel slices of the weight gradient: whole 64-pixel steps, enough slices
-// for about 2048 workgroups in all (1..256), at least 512 pixels each
-void slices(int64_t P, int Cin, int Cout, int64_t* chunk, int* S) {
+// Work split of the weight gradient.  Thin layers (Cout <= 32) go to
+// conv_bwd_w_small with slices of row segments, about 768 workgroups in all
+// (three per CU fit its LDS); the others to conv_bwd_w with pixel slices of
+// whole 64-pixel steps, about 2048 workgroups (1..256 slices), at least 512
+// pixels each.  chunk counts segments or pixels.
+struct BwdWPlan {
+    bool small;
+    int64_t chunk;
+    int S;
+};
+
+BwdWPlan bwd_w_plan(int N, int H, int W, int Cin, int Cout) {
+    BwdWPlan pl{};
+    if (Cout <= kSmallCo) {
+        const int64_t nseg = (int64_t)N * H * ((W + 63) / 64);
+        const int64_t nb = (Cin + kSmallCi - 1) / kSmallCi;
+        const int64_t want = std::min<int64_t>(nseg, std::max<int64_t>(1, (768 + nb - 1) / nb));
+        pl.small = true;
+        pl.chunk = (nseg + want - 1) / want;
+        pl.S = (int)((nseg + pl.chunk - 1) / pl.chunk);
+        return pl;
+    }
+    const int64_t P = (int64_t)N * H * W;
     const int64_t per_slice = (int64_t)((Cout + 63) / 64) * 9 * ((Cin + 63) / 64);
     const int64_t want = std::min<int64_t>(256, std::max<int64_t>(1, (2048 + per_slice - 1) / per_slice));
     int64_t c = (P + want - 1) / want;
     c = std::max<int64_t>((c + kStep - 1) / kStep * kStep, 512);
-    *chunk = c;
-    *S = (int)((P + c - 1) / c);
+    pl.small = false;
+    pl.chunk = c;
+    pl.S = (int)((P + c - 1) / c);
+    return pl;
 }
 
 }  // namespace
@@ -360,10 +515,8 @@ int fds_conv3x3_bwd_data(const uint16_t* dy, const uint16_t* wt, const uint16_t*
 }
 
 int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
-    int64_t chunk;
-    int S;
-    slices((int64_t)N * H * W, Cin, Cout, &chunk, &S);
-    return (int64_t)S * Cout * (9 * (int64_t)Cin + 1);
+    const BwdWPlan pl = bwd_w_plan(N, H, W, Cin, Cout);
+    return (int64_t)pl.S * Cout * (9 * (int64_t)Cin + 1);
 }
 
 int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t* relu_y, float* dw, float* dbias,
@@ -372,15 +525,20 @@ int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t
     if (rc) return rc;
     if (Cout % 8) return fail(FDS_E_ARG, "Cout must be a multiple of 8");
     if (!x || !dy || !dw || !scratch) return fail(FDS_E_ARG, "NULL pointer");
-    const int64_t P = (int64_t)N * H * W;
-    int64_t chunk;
-    int S;
-    slices(P, Cin, Cout, &chunk, &S);
+    const BwdWPlan pl = bwd_w_plan(N, H, W, Cin, Cout);
+    const int S = pl.S;
     hipStream_t st = (hipStream_t)stream;
     const int64_t nw = (int64_t)Cout * 9 * Cin;
     float* bp = dbias ? scratch + (int64_t)S * nw : nullptr;
-    const dim3 grid((unsigned)S, (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
-    hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout, chunk);
+    if (pl.small) {
+        const dim3 grid((unsigned)S, (unsigned)((Cin + kSmallCi - 1) / kSmallCi));
+        hipLaunchKernelGGL(conv_bwd_w_small, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout,
+                           pl.chunk);
+    } else {
+        const dim3 grid((unsigned)S, (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
+        hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout,
+                           pl.chunk);
+    }
     FDS_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, scratch, dw, nw, S);
     FDS_HIP(hipGetLastError());
