@@ -104,6 +104,9 @@ def main(argv=None):
     ap.add_argument("--dagger-iters", type=int, default=4000)
     ap.add_argument("--teacher", action="store_true",
                     help="diagnostic: jumps to the plain run's own coarse psi (perfect coarse predictor)")
+    ap.add_argument("--extra-seeds", default="",
+                    help="further networks (initialisation and batch order seeds, e.g. 1,2,3) trained on the "
+                         "same data and run through the same accelerated runs: the spread over training")
     ap.add_argument("--workload-seed", type=int, default=None,
                     help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
@@ -112,6 +115,8 @@ def main(argv=None):
     ap.add_argument("--velocity", type=float, default=0.0,
                     help="scale of the predicted velocity change applied at each jump (0: keep the solver's)")
     a = ap.parse_args(argv)
+    if a.extra_seeds and a.load_model:
+        ap.error("--extra-seeds trains on generated data; it cannot be combined with --load-model")
     sg.LAMBDA_S = a.phase_weight
     os.makedirs(a.out, exist_ok=True)
     res = {"args": vars(a)}
@@ -153,8 +158,9 @@ def main(argv=None):
             log("held-out after coupled pairs: %s" % json.dumps(res["eval_heldout_after_dagger"]))
             del cp
         save_model(model, os.path.join(a.out, "unet.npz"))
-        del data
-        torch.cuda.empty_cache()
+        if not a.extra_seeds:
+            del data
+            torch.cuda.empty_cache()
     res["mfma"] = mfma_throughput(model)
     log("mfma: %s" % json.dumps(res["mfma"]))
 
@@ -182,29 +188,42 @@ def main(argv=None):
         if v:
             r, s, b = (v.split(":") + [str(a.band)])[:3]
             runs.append((int(r), float(s), int(b)))
-    for relax, vel, band in runs:
-        # the plain run's coarse snapshots sit at chunk + k * (jump + relax)
-        # only for the main relax length; other variants compare at breakthrough
-        cm = Compute(mask, links, params)
-        t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, model, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
-                                                       relax, a.threshold, a.max_steps,
-                                                       mass_fix=not a.no_mass_fix, velocity=vel, band=band,
-                                                       teacher=rec_p if a.teacher else None, log=log)
-        acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
-                   non_finite=cm.mass()[2])
-        cm.close()
-        if "plain" in res:
-            p = res["plain"]
-            acc["speedup_wall"] = p["wall_s"] / wall_a if wall_a > 0 else None
-            acc["breakthrough_t_rel_error"] = (abs(t_a - p["breakthrough_t"]) / p["breakthrough_t"]
-                                               if t_a and p["breakthrough_t"] else None)
-            common = sorted(set(rec_p) & set(rec_a))
-            errs = [float(sg.interface_error(rec_a[t], rec_p[t], fluid_c)) for t in common]
-            acc["interface_error_vs_plain_mean"] = float(np.mean(errs)) if errs else None
-            acc["interface_error_vs_plain_last"] = errs[-1] if errs else None
-            acc["compared_at"] = len(common)
-        log("accelerated: %s" % json.dumps(acc))
-        res.setdefault("accelerated", []).append(acc)
+
+    def run_accelerated(net):
+        out = []
+        for relax, vel, band in runs:
+            # the plain run's coarse snapshots sit at chunk + k * (jump + relax)
+            # only for the main relax length; other variants compare at breakthrough
+            cm = Compute(mask, links, params)
+            t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, net, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
+                                                           relax, a.threshold, a.max_steps,
+                                                           mass_fix=not a.no_mass_fix, velocity=vel, band=band,
+                                                           teacher=rec_p if a.teacher else None, log=log)
+            acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
+                       non_finite=cm.mass()[2])
+            cm.close()
+            if "plain" in res:
+                p = res["plain"]
+                acc["speedup_wall"] = p["wall_s"] / wall_a if wall_a > 0 else None
+                acc["breakthrough_t_rel_error"] = ((t_a - p["breakthrough_t"]) / p["breakthrough_t"]
+                                                   if t_a and p["breakthrough_t"] else None)
+                common = sorted(set(rec_p) & set(rec_a))
+                errs = [float(sg.interface_error(rec_a[t], rec_p[t], fluid_c)) for t in common]
+                acc["interface_error_vs_plain_mean"] = float(np.mean(errs)) if errs else None
+                acc["interface_error_vs_plain_last"] = errs[-1] if errs else None
+                acc["compared_at"] = len(common)
+            log("accelerated: %s" % json.dumps(acc))
+            out.append(acc)
+        return out
+
+    res["accelerated"] = run_accelerated(model)
+    for sd in [int(v) for v in a.extra_seeds.split(",") if v]:
+        net = sg.UNet(seed=sd).cuda()
+        hist_s, secs_s = sg.train(net, data, tr, a.jump, a.snap, iters=a.iters, bs=a.bs, lr=a.lr, seed=sd, log=log)
+        ev = sg.evaluate(net, data, te, a.jump, a.snap)
+        net.eval()
+        res.setdefault("seeds", []).append({"seed": sd, "train_seconds": secs_s, "eval_heldout": ev,
+                                            "accelerated": run_accelerated(net)})
     log(json.dumps(res, indent=1, default=float))
     with open(os.path.join(a.out, "surrogate_run.json"), "w") as fh:
         json.dump(res, fh, indent=1, default=float)
