@@ -51,6 +51,17 @@ def load_model(model, path):
     return model
 
 
+class MeanNet(torch.nn.Module):
+    """The mean of several networks' outputs (an ensemble surrogate)."""
+
+    def __init__(self, nets):
+        super().__init__()
+        self.nets = torch.nn.ModuleList(nets)
+
+    def forward(self, x):
+        return torch.stack([n(x) for n in self.nets]).mean(dim=0)
+
+
 def mfma_throughput(model, H=128, W=256, bs=64, iters=20):
     """Forward + backward of the U-Net on a random batch: TFLOP/s of the
     convolutions (3x the forward FLOPs) against the bf16 dense peak."""
@@ -107,6 +118,8 @@ def main(argv=None):
     ap.add_argument("--extra-seeds", default="",
                     help="further networks (initialisation and batch order seeds, e.g. 1,2,3) trained on the "
                          "same data and run through the same accelerated runs: the spread over training")
+    ap.add_argument("--ensemble-eval", action="store_true",
+                    help="with --extra-seeds: also run the accelerated runs with the mean of all the networks")
     ap.add_argument("--workload-seed", type=int, default=None,
                     help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
@@ -217,13 +230,19 @@ def main(argv=None):
         return out
 
     res["accelerated"] = run_accelerated(model)
+    nets = [model]
     for sd in [int(v) for v in a.extra_seeds.split(",") if v]:
         net = sg.UNet(seed=sd).cuda()
         hist_s, secs_s = sg.train(net, data, tr, a.jump, a.snap, iters=a.iters, bs=a.bs, lr=a.lr, seed=sd, log=log)
         ev = sg.evaluate(net, data, te, a.jump, a.snap)
         net.eval()
+        nets.append(net)
         res.setdefault("seeds", []).append({"seed": sd, "train_seconds": secs_s, "eval_heldout": ev,
                                             "accelerated": run_accelerated(net)})
+    if a.ensemble_eval and len(nets) > 1:
+        mean = MeanNet(nets).eval()
+        res["ensemble"] = {"networks": len(nets), "eval_heldout": sg.evaluate(mean, data, te, a.jump, a.snap),
+                           "accelerated": run_accelerated(mean)}
     log(json.dumps(res, indent=1, default=float))
     with open(os.path.join(a.out, "surrogate_run.json"), "w") as fh:
         json.dump(res, fh, indent=1, default=float)
