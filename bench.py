@@ -196,8 +196,8 @@ def main():
         ensemble_bench(args.ensemble, args.steps if args.steps != 20000 else 2000, args.warmup if args.warmup != 500 else 20)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(x-slab sharding with halo exchange is a design, not built)", file=sys.stderr)
         sys.exit(2)
     # single process: any launcher-provided rank other than 0 has nothing to do
     if int(os.environ.get("RANK", "0")) != 0:
