@@ -575,11 +575,12 @@ def jump_change(pc, dpsi, fluid_c, n_c, band, inflow=None):
 
 
 @torch.no_grad()
-def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
+def plain_run(cm, f, max_steps, chunk=200, record_every=None, velocity_record=None, log=print):
     """Plain LBM from cm's current (fresh) state until breakthrough at the
     probe column, checked every `chunk` iterations on device arrays.  With
     record_every, the coarse psi of every multiple of it (offset by chunk)
-    is kept for comparisons.  Returns (breakthrough iteration or None, wall
+    is kept for comparisons (and the coarse ux, uy in velocity_record, a
+    dict, when given).  Returns (breakthrough iteration or None, wall
     seconds, {iteration: coarse psi})."""
     col = probe_column(cm.W)
     rec = {}
@@ -594,6 +595,8 @@ def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
             buf = cm.fields_device(buf)
             if record_every and (cm.t - chunk) % record_every == 0:
                 rec[cm.t] = coarsen(buf[0], fluid, f)
+                if velocity_record is not None:
+                    velocity_record[cm.t] = coarsen(buf[[2, 3]], fluid, f)
             hit, bad = status(buf, fluid, col)
             if bad:
                 log("plain: non-finite at t=%d" % cm.t)
@@ -609,7 +612,7 @@ def plain_run(cm, f, max_steps, chunk=200, record_every=None, log=print):
 
 @torch.no_grad()
 def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=0.02, max_steps=10 ** 7,
-                    mass_fix=True, velocity=0.0, band=2, teacher=None, log=print):
+                    mass_fix=True, velocity=0.0, band=2, teacher=None, teacher_velocity=None, log=print):
     """Surrogate-accelerated run of solver cm (freshly initialised) until
     breakthrough, every array on the device and every launch on the
     solver's stream.
@@ -632,7 +635,9 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
     iterations instead.  Breakthrough is checked after every cycle.
     With `teacher` ({iteration: coarse psi} of a plain run) the jump's
     target is the plain run's coarse psi K iterations later instead of the
-    network's (a diagnostic of the restart with a perfect coarse predictor).
+    network's (a diagnostic of the restart with a perfect coarse predictor);
+    with `teacher_velocity` ({iteration: coarse ux, uy}) the velocity change
+    applied (times `velocity`) is the plain run's too.
     Returns (breakthrough model time or None, wall seconds, stats,
     {model time: coarse psi})."""
     H, W = cm.H, cm.W
@@ -670,6 +675,10 @@ def accelerated_run(cm, model, M, theta_inv, f, jump=2000, relax=200, threshold=
             if teacher is not None and t_model + jump in teacher:
                 out = out.clone()
                 out[..., 0] = teacher[t_model + jump] - pc
+                if teacher_velocity is not None and t_model + jump in teacher_velocity:
+                    tu = teacher_velocity[t_model + jump]
+                    out[..., 1] = (tu[0] - uxc) / U_REF
+                    out[..., 2] = (tu[1] - uyc) / U_REF
             delta, scale = jump_change(pc, out[..., 0], fluid_c, n_c, band, inflow if mass_fix else None)
             if mass_fix:
                 stats["mass_scale"].append(scale)

@@ -113,6 +113,8 @@ def main(argv=None):
                     help="coupled-loop cycles on the training flows whose states, labelled by the solver, "
                          "fine-tune the network (0: none)")
     ap.add_argument("--dagger-iters", type=int, default=4000)
+    ap.add_argument("--teacher-velocity", action="store_true",
+                    help="with --teacher: the jumps' velocity change is the plain run's too (use --velocity 1)")
     ap.add_argument("--teacher", action="store_true",
                     help="diagnostic: jumps to the plain run's own coarse psi (perfect coarse predictor)")
     ap.add_argument("--extra-seeds", default="",
@@ -184,11 +186,12 @@ def main(argv=None):
     res["workload"] = dict(spec, porosity_reached=float(mask.mean()), psi_wall=params.psi_wall, u0=params.u0,
                            probe_column=sg.probe_column(spec["W"]))
     cycle = a.jump + a.relax
-    rec_p = {}
+    rec_p, rec_u = {}, ({} if a.teacher_velocity else None)
     if not a.skip_plain:
         cm = Compute(mask, links, params)
         t_b, wall, rec_p = sg.plain_run(cm, a.f, a.max_steps, chunk=a.relax,
-                                        record_every=a.relax if a.teacher else cycle, log=log)
+                                        record_every=a.relax if a.teacher else cycle, velocity_record=rec_u,
+                                        log=log)
         res["plain"] = {"breakthrough_t": t_b, "wall_s": wall, "mlups": cm.t * mask.size / wall / 1e6,
                         "non_finite": cm.mass()[2]}
         cm.close()
@@ -211,7 +214,8 @@ def main(argv=None):
             t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, net, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
                                                            relax, a.threshold, a.max_steps,
                                                            mass_fix=not a.no_mass_fix, velocity=vel, band=band,
-                                                           teacher=rec_p if a.teacher else None, log=log)
+                                                           teacher=rec_p if a.teacher else None,
+                                                           teacher_velocity=rec_u, log=log)
             acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
                        non_finite=cm.mass()[2])
             cm.close()
