@@ -96,3 +96,12 @@ def test_model_file_round_trip(tmp_path):
     load_model(b, path)
     for (ka, va), (kb, vb) in zip(a.state_dict().items(), b.state_dict().items()):
         assert ka == kb and torch.equal(va, vb)
+
+
+def test_pipeline_rejects_extra_seeds_with_a_loaded_model(tmp_path):
+    """--extra-seeds trains on generated data, so it cannot follow
+    --load-model; the parser refuses before anything touches the GPU."""
+    import pytest
+    from fingering_dynamics_amd import surrogate_run
+    with pytest.raises(SystemExit):
+        surrogate_run.main(["--out", str(tmp_path), "--extra-seeds", "1", "--load-model", "w.npz"])
