@@ -584,9 +584,9 @@ __global__ __launch_bounds__(64 * WPB) void ens_kernel(const EnsDomain* __restri
 //            fields + partial sums.
 //   PRE = 0: post-collision populations of iteration 0 into b.dst and the
 //            side buffer P of the initial psi into b.side_out.
-//   ux0/uy0 (PRE = 0 only, may be NULL): restart from given fields - the
-//            velocity is taken as given and mu from psi, so a running
-//            flow continues (fd_init_fields).
+//   ux0/uy0 (may be NULL): restart from given fields - the velocity is
+//            taken as given and mu from psi, so a running flow continues
+//            (fd_init_fields); with PRE = 1 the N = 0 state of such a start.
 template <int PRE>
 __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const float* __restrict__ psi0,
                                                    const float* __restrict__ rho0, int walled_init,
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             m.ux = 0.f;
             m.uy = walled_init ? (0.5f * m.fy) / m.rho : 0.f;
             m.p = m.rho * (1.f / 3.f);  // evaluated before mu (both inits)
-            if (!PRE && ux0) {
+            if (ux0) {
                 m.mu = fdn::chem_pot(psi, lap, k.c);
                 m.fx = m.mu * dpx + k.c.gx;
                 m.fy = m.mu * dpy + k.c.gy;
@@ -728,6 +728,7 @@ struct fd_handle {
     int cur = 0, sc = 0;
     int64_t N = 0;
     bool inited = false;
+    bool init_u = false;  // started by fd_init_fields*: the velocity is staged in fields planes 2, 3
     int wx = 4;     // waves (64-column strips) per workgroup
     int rows = 16;  // rows per wave
     int timed_launches = 0;
@@ -918,8 +919,13 @@ int compute_fields(fd_handle* h, float* pops) {
         if (rc) return rc;
         b.partials = h->partials;
         if (!pops) b.dst = h->pop[1 - h->cur];
+        // after fd_init_fields the staged velocity (fields planes 2, 3) is
+        // part of the N = 0 state; each node reads its own value before
+        // writing the same one back
+        const float* ux0 = h->init_u ? h->fields + 2 * h->Q : nullptr;
+        const float* uy0 = h->init_u ? h->fields + 3 * h->Q : nullptr;
         hipLaunchKernelGGL((init_kernel<1>), grid, dim3(64, 4), 0, h->stream, h->k, b, h->psi0, h->rho0,
-                           h->p.init_mode == FD_INIT_WALLED ? 1 : 0);
+                           h->p.init_mode == FD_INIT_WALLED ? 1 : 0, ux0, uy0);
         FD_HIP(hipGetLastError());
     } else {
         nblk = moments_grid_blocks(h);
@@ -1102,6 +1108,7 @@ int init_impl(fd_handle* h, const double* psi0, const double* rho0, const double
     h->sc = 0;
     h->N = 0;
     h->inited = true;
+    h->init_u = ux0 != nullptr;
     h->have_timing = false;
     return FD_OK;
 }
@@ -1397,6 +1404,7 @@ int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, cons
     h->cur = 0;
     h->sc = 0;
     h->N = 0;
+    h->init_u = true;
     h->have_timing = false;
     return FD_OK;
 }

@@ -120,6 +120,34 @@ def test_restart_from_fields_continues_the_flow():
         s_.close()
 
 
+def test_fields_right_after_a_restart_are_the_given_fields():
+    """Before the first iteration after fd_init_fields(_device), the fields
+    read back are the ones handed over (velocity included), and a plain
+    fd_init afterwards reads back at rest again."""
+    from fingering_dynamics_amd.lbm import Compute
+    from fingering_dynamics_amd.ensemble import domain_spec, build_domain
+    s = domain_spec(3)
+    s.update(H=64, W=192)
+    m, l, p = build_domain(s)
+    a = Compute(m, l, p)
+    a.step(500)
+    psi, rho, ux, uy = [v.astype(np.float64) for v in a.fields()]
+    b = Compute(m, l, p)
+    b.init_fields(psi, rho, ux, uy)
+    got = b.fields()
+    for x, y in zip(got, (psi, rho, ux, uy)):
+        assert np.array_equal(x[m], y[m].astype(np.float32))
+    assert np.abs(ux[m]).max() > 0
+    c = Compute(m, l, p)
+    with torch.cuda.stream(c.torch_stream()):
+        c.init_fields_device(*[torch.as_tensor(v, dtype=torch.float32).cuda() for v in (psi, rho, ux, uy)])
+    assert np.array_equal(c.fields()[2][m], ux[m].astype(np.float32))
+    b.init(psi, rho)
+    assert not b.fields()[2].any()
+    for s_ in (a, b, c):
+        s_.close()
+
+
 def test_device_coupling_matches_host_path():
     """fd_fields_device / fd_init_fields_device / device state blobs give
     bit-for-bit what the host-array entry points give."""
