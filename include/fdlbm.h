@@ -101,8 +101,10 @@ int fd_init(fd_handle* h, const double* psi0, const double* rho0);
 /* Restart from given fields, double[H*W] each: the equilibrium state of
  * psi, rho and velocity (ux, uy) with mu and the force mu grad psi taken
  * from psi, collided once - the state a running flow had, with a new phase
- * field.  Resets the step counter to 0.  No reference counterpart: used
- * where a learned surrogate hands the phase field back to the solver. */
+ * field.  Resets the step counter to 0; reads before the first fd_step
+ * return the given fields (and their equilibria as populations).  No
+ * reference counterpart: used where a learned surrogate hands the phase
+ * field back to the solver. */
 int fd_init_fields(fd_handle* h, const double* psi, const double* rho, const double* ux, const double* uy);
 
 /* Advance nsteps iterations of the main loop body
