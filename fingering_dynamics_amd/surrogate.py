@@ -45,6 +45,8 @@ U_REF = 1e-3          # velocity scale of the inputs (the workload's inlet veloc
 LAMBDA_U = 0.01       # weight of the velocity change in the training loss
 LAMBDA_S = 0.0        # weight of the phase (sign) cross-entropy on interface cells (0.01 tried)
 SIGN_GAIN = 4.0       # logit = SIGN_GAIN * predicted psi
+LOSS_BAND = 0         # > 0: the psi and velocity MSE count only cells within this many coarse cells of
+                      # the interface at t, the cells a jump may change (0: every fluid cell)
 CHANNELS = (32, 64, 128, 128)
 IN_CH = 8
 
@@ -373,6 +375,8 @@ def loss_fn(model, x, psi0, psi1, v0, v1, wfluid):
     makes the one-step phase prediction decisive, but did not shorten the
     coupled run's breakthrough, so it is off by default)."""
     out = model(x)
+    if LOSS_BAND > 0:
+        wfluid = wfluid * near_interface(psi0, LOSS_BAND)
     w = wfluid.sum().clamp(min=1.0)
     pred = psi0 + out[..., 0]
     lp = ((pred - psi1) ** 2 * wfluid).sum() / w
@@ -555,6 +559,14 @@ def batch_pairs(cp, idx, flip=None):
 
 
 # ------------------------------------------------------- plain and accelerated runs
+def near_interface(pc, band):
+    """1.0 on cells within `band` cells (Chebyshev distance, no wrap) of an
+    interface cell (|pc| < 0.9), else 0.0; pc is [H, W] or [N, H, W]."""
+    m = (pc.abs() < 0.9).float()
+    m4 = m.reshape(-1, 1, *m.shape[-2:])
+    return torch.nn.functional.max_pool2d(m4, 2 * band + 1, stride=1, padding=band).reshape(m.shape)
+
+
 def jump_change(pc, dpsi, fluid_c, n_c, band, inflow=None):
     """The coarse psi change a jump applies: the predicted change dpsi
     (with the prediction clamped to [-1, 1]) on fluid cells within `band`
@@ -563,9 +575,7 @@ def jump_change(pc, dpsi, fluid_c, n_c, band, inflow=None):
     Returns (change, scale)."""
     delta = ((pc + dpsi).clamp(-1.0, 1.0) - pc) * fluid_c
     if band > 0:
-        near = (pc.abs() < 0.9).float()[None, None]
-        near = torch.nn.functional.max_pool2d(near, 2 * band + 1, stride=1, padding=band)[0, 0]
-        delta = delta * near
+        delta = delta * near_interface(pc, band)
     scale = torch.ones((), device=pc.device)
     if inflow is not None:
         gain = (delta * n_c).sum()

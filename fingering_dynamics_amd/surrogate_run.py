@@ -126,6 +126,8 @@ def main(argv=None):
                     help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
                     help="weight of the phase cross-entropy term of the training loss")
+    ap.add_argument("--loss-band", type=int, default=sg.LOSS_BAND,
+                    help="train on cells within this many coarse cells of the interface only (0: all)")
     ap.add_argument("--regime", default="target", choices=sorted(sg.REGIMES))
     ap.add_argument("--velocity", type=float, default=0.0,
                     help="scale of the predicted velocity change applied at each jump (0: keep the solver's)")
@@ -133,6 +135,7 @@ def main(argv=None):
     if a.extra_seeds and a.load_model:
         ap.error("--extra-seeds trains on generated data; it cannot be combined with --load-model")
     sg.LAMBDA_S = a.phase_weight
+    sg.LOSS_BAND = a.loss_band
     os.makedirs(a.out, exist_ok=True)
     res = {"args": vars(a)}
     model = sg.UNet(seed=0).cuda()

@@ -55,6 +55,23 @@ def test_interface_error_counts_fluid_cells_of_other_sign():
     assert float(sg.interface_error(a, a, fc)) == 0.0
 
 
+def test_near_interface_is_the_dilated_interface():
+    """Brute force: a cell is near when some interface cell (|psi| < 0.9)
+    lies within `band` rows and columns of it."""
+    rng = np.random.default_rng(7)
+    pc = torch.as_tensor(np.sign(rng.standard_normal((2, 9, 13))) * (rng.random((2, 9, 13)) > 0.1),
+                         dtype=torch.float32)
+    iface = pc.abs() < 0.9
+    for band in (1, 2):
+        got = sg.near_interface(pc, band)
+        assert got.shape == pc.shape and torch.equal(sg.near_interface(pc[1], band), got[1])
+        for n in range(2):
+            for i in range(9):
+                for j in range(13):
+                    w = iface[n, max(0, i - band):i + band + 1, max(0, j - band):j + band + 1]
+                    assert float(got[n, i, j]) == float(w.any())
+
+
 def test_flow_specs():
     s1, s2 = sg.flow_spec(5), sg.flow_spec(5)
     assert s1 == s2
