@@ -897,6 +897,7 @@ int ensure_partials(fd_handle* h, int n) {
     if (n <= h->npart) return FD_OK;
     if (h->partials) FD_HIP(hipFree(h->partials));
     h->partials = nullptr;
+    h->npart = 0;  // a failed allocation below must not leave a stale capacity behind
     FD_HIP(hipMalloc(&h->partials, sizeof(double) * 3 * (size_t)n));
     h->npart = n;
     return FD_OK;
