@@ -122,3 +122,29 @@ def test_pipeline_rejects_extra_seeds_with_a_loaded_model(tmp_path):
     from fingering_dynamics_amd import surrogate_run
     with pytest.raises(SystemExit):
         surrogate_run.main(["--out", str(tmp_path), "--extra-seeds", "1", "--load-model", "w.npz"])
+
+
+def test_jump_change_clamps_bands_and_scales_to_the_inflow():
+    """The coarse change of one jump: the prediction clamped so psi stays in
+    [-1, 1], zero on solid cells and beyond `band` cells of the interface,
+    and scaled so that sum(change * n_c) equals the inflow, with the scale
+    held to [0.5, 2]."""
+    pc = torch.tensor([[1.0, 1.0, 0.2, -1.0, -1.0, -1.0, -1.0]] * 2)
+    dpsi = torch.tensor([[0.5, 0.0, 0.3, 0.4, 0.2, 0.1, 0.1]] * 2)
+    fluid = torch.ones_like(pc)
+    fluid[1, 3] = 0.0
+    n_c = torch.full_like(pc, 64.0)
+    d, s = sg.jump_change(pc, dpsi, fluid, n_c, band=0)
+    assert float(s) == 1.0
+    want = torch.tensor([[0.0, 0.0, 0.3, 0.4, 0.2, 0.1, 0.1], [0.0, 0.0, 0.3, 0.0, 0.2, 0.1, 0.1]])
+    assert torch.allclose(d, want)                           # 1 + 0.5 clamps to 1: no change
+    d1, _ = sg.jump_change(pc, dpsi, fluid, n_c, band=1)    # interface cell: column 2 only
+    assert torch.allclose(d1, want * (torch.arange(7) <= 3).float())
+    gain = float((d1 * n_c).sum())
+    d2, s2 = sg.jump_change(pc, dpsi, fluid, n_c, band=1, inflow=torch.tensor(1.5 * gain))
+    assert abs(float(s2) - 1.5) < 1e-6 and abs(float((d2 * n_c).sum()) - 1.5 * gain) < 1e-3
+    _, s3 = sg.jump_change(pc, dpsi, fluid, n_c, band=1, inflow=torch.tensor(10.0 * gain))
+    _, s4 = sg.jump_change(pc, dpsi, fluid, n_c, band=1, inflow=torch.tensor(0.1 * gain))
+    assert float(s3) == 2.0 and float(s4) == 0.5
+    _, s5 = sg.jump_change(pc, -dpsi.abs() - 2.0, fluid, n_c, band=1, inflow=torch.tensor(5.0))
+    assert float(s5) == 1.0                                  # no gain: nothing to scale
