@@ -1,6 +1,8 @@
 """Host-side pieces of the surrogate (CPU tensors): coarsening and its
 inverse, the interface error, the flow generators, the network's parameter
-and FLOP counts, and the weight file round trip."""
+and FLOP counts, the weight file round trip, the network input layout, the
+jump hand-off arithmetic, the breakthrough probe, the training pair index,
+and the many-domain ensemble's generators."""
 import math
 
 import numpy as np
@@ -195,3 +197,32 @@ def test_pairs_keep_t_plus_jump_among_the_usable_snapshots():
     assert items == [(0, t) for t in range(6)]        # domain 1 has 4 usable snapshots: t + 4 never fits
     items, _ = sg.pairs(data, [1], jump=1000, snap=500)
     assert items == [(1, 0), (1, 1)]
+
+
+def test_ensemble_domain_specs_and_square_beds():
+    """The many-domain workload's specs stay in their stated ranges and are
+    reproducible; square beds keep a one-node gap between squares, stay
+    inside their column and row bounds and never exceed the target solid
+    fraction; a built domain carries the spec's flow parameters."""
+    from fingering_dynamics_amd import ensemble, wetting
+    for k in range(40):
+        s = ensemble.domain_spec(k)
+        assert s == ensemble.domain_spec(k) and s != ensemble.domain_spec(k, seed=4)
+        assert s["W"] % 32 == 0 and 256 <= s["W"] <= 1024 and s["H"] % 32 == 0 and 128 <= s["H"] <= 512
+        assert 0.6 <= s["porosity"] <= 0.85 and 1.0 <= s["M"] <= 20.0 and 31.0 <= s["theta"] <= 90.0
+    H, W, size = 64, 200, 6
+    bed = ensemble._square_bed(H, W, 0.8, seed=9, x_lo=24, x_hi=W - 25, size=size)
+    assert 0 < bed.sum() <= 0.2 * H * W and bed.sum() % (size * size) == 0
+    ys, xs = np.nonzero(bed)
+    assert xs.min() >= 24 and xs.max() < W - 25 and ys.min() >= 2 and ys.max() <= H - 3
+    # squares: flood-label the bed and check each component is one size x size block
+    from scipy import ndimage
+    lab, n = ndimage.label(bed, structure=np.ones((3, 3)))   # 8-connectivity: touching corners merge
+    assert n * size * size == bed.sum()
+    spec = next(ensemble.domain_spec(k) for k in range(40) if not ensemble.domain_spec(k)["circles"]
+                and ensemble.domain_spec(k)["H"] * ensemble.domain_spec(k)["W"] <= 256 * 512)
+    mask, links, p = ensemble.build_domain(spec)
+    assert mask.shape == (spec["H"], spec["W"]) and links.shape == mask.shape
+    assert p.H == spec["H"] and p.W == spec["W"] and p.M == spec["M"] and p.u0 == 0.001
+    assert p.wall_lo == 0 and p.wall_hi == spec["H"] - 1
+    assert p.psi_wall == wetting.psi_wall_for_angle(180.0 - spec["theta"])
