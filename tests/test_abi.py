@@ -107,3 +107,38 @@ def test_surrogate_library_exports_its_header():
     from fingering_dynamics_amd import surrogate_ops
     assert set(surrogate_ops.EXPORTS) == set(names)
     surrogate_ops.load()
+
+
+def test_surrogate_library_rejects_bad_arguments_before_any_launch():
+    """Shape and pointer checks of libfdsurr.so return -1 with a message
+    before any HIP call, so they run without a GPU.  Every call here is
+    rejected; none reaches a kernel."""
+    if not os.path.exists(SURR_LIB):
+        from fingering_dynamics_amd import build
+        build.build_surrogate()
+    from fingering_dynamics_amd import surrogate_ops
+    lib = surrogate_ops.load()
+    err = lambda: lib.fds_last_error().decode()
+    cases = [
+        (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 0, 8, 8, 8, 8, 1, None), "bad shape"),
+        (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 1, 8, 8, 12, 8, 1, None), "Cin must be"),
+        (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 1, 8, 8, 8, 12, 1, None), "Cout must be"),
+        (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 1, 8, 8, 8, 8, 1, None), "NULL pointer"),
+        (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 1 << 20, 1 << 10, 1 << 10, 8, 8, 1, None),
+         "too many pixels"),
+        (lambda: lib.fds_conv3x3_bwd_data(None, None, None, None, None, 1, 8, 8, 16, 8, None), "NULL pointer"),
+        (lambda: lib.fds_conv3x3_bwd_data(None, None, None, None, None, 1, 8, 8, 8, 4, None), "bad shape"),
+        (lambda: lib.fds_conv3x3_bwd_weight(None, None, None, None, None, None, 1, 8, 8, 8, 12, None),
+         "Cout must be"),
+        (lambda: lib.fds_conv3x3_bwd_weight(None, None, None, None, None, None, 1, 8, 8, 8, 8, None),
+         "NULL pointer"),
+        (lambda: lib.fds_flip_weights(None, None, 8, 8, None), "bad arguments"),
+    ]
+    for call, msg in cases:
+        assert call() == -1
+        assert msg in err(), (msg, err())
+    # the weight-gradient scratch holds S slices of Cout x (9 Cin + 1) floats
+    for shape in [(64, 128, 256, 8, 32), (64, 32, 64, 128, 128), (1, 8, 8, 8, 8)]:
+        n = lib.fds_bwd_weight_scratch(*shape)
+        per = shape[4] * (9 * shape[3] + 1)
+        assert n >= per and n % per == 0
