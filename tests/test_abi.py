@@ -142,3 +142,44 @@ def test_surrogate_library_rejects_bad_arguments_before_any_launch():
         n = lib.fds_bwd_weight_scratch(*shape)
         per = shape[4] * (9 * shape[3] + 1)
         assert n >= per and n % per == 0
+
+
+def test_solver_rejects_bad_parameters_before_touching_the_device(lib_path):
+    """fd_create checks its parameters first (-1 and a message, no HIP
+    call), so the checks run without a GPU.  Every case is invalid; none
+    creates a handle."""
+    import numpy as np
+    from fingering_dynamics_amd import _native
+    from fingering_dynamics_amd.lbm import _to_fd_params
+    from fingering_dynamics_amd.params import LbmParams
+    lib = _native.load()
+    solid = np.zeros(64 * 64, np.uint8)
+    links = np.zeros(64 * 64, np.uint8)
+    cases = [
+        (dict(H=2), "H and W must be >= 3"),
+        (dict(W=2), "H and W must be >= 3"),
+        (dict(H=8192, W=4096), "domain too large"),
+        (dict(y_bc=2), "bad y_bc"),
+        (dict(x_bc=-1), "bad x_bc"),
+        (dict(profile=2), "bad profile"),
+        (dict(outlet=2), "bad outlet"),
+        (dict(init_mode=2), "bad init_mode"),
+        (dict(H=63, x_bc=0, profile=0), "Gaussian inlet profile needs an even H"),
+        (dict(y_bc=1, wall_lo=-1), "wall rows outside"),
+        (dict(y_bc=1, wall_hi=64), "wall rows outside"),
+        (dict(tau=0.5), "bad tau/eta_n/M"),
+        (dict(eta_n=0.0), "bad tau/eta_n/M"),
+        (dict(M=float("nan")), "bad tau/eta_n/M"),
+    ]
+    for change, msg in cases:
+        fdp = _to_fd_params(LbmParams.periodic(64, 64))
+        for k, v in change.items():
+            setattr(fdp, k, v)
+        h = ctypes.c_void_p()
+        rc = lib.fd_create(ctypes.byref(fdp), _native.ptr(solid), _native.ptr(links), ctypes.byref(h))
+        msg_got = lib.fd_last_error().decode()
+        assert rc == -1 and msg in msg_got, (change, rc, msg_got)
+        assert not h.value
+    h = ctypes.c_void_p()
+    assert lib.fd_create(None, _native.ptr(solid), _native.ptr(links), ctypes.byref(h)) == -1
+    assert "params is NULL" in lib.fd_last_error().decode()
