@@ -40,7 +40,9 @@ EXPORTS = {
     "fd_get_fields": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "fd_get_fields_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 4),
     "fd_get_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fd_set_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_phase_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fd_watchdog": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fd_state_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "fd_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),

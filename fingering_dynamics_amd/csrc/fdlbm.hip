@@ -213,47 +213,64 @@ __device__ __forceinline__ void bc_f(const KParams& k, const KBufs& b, rsrc_t sr
     }
 }
 
-template <int NT>
-__device__ __forceinline__ void block_sum3(double s0, double s1, double s2, double* out) {
-    __shared__ double sm[3][NT / 64];
+// FP64 reduction of the per-node health sums: s0 = sum rho, s1 = sum psi,
+// s2 = non-finite count (sums), m3 = min(tau_mix - 1/2) (minimum).
+struct Red4 {
+    double s0, s1, s2, m3;
+};
+
+__device__ __forceinline__ Red4 red4_init() { return Red4{0.0, 0.0, 0.0, HUGE_VAL}; }
+
+__device__ __forceinline__ Red4 wave_red4(Red4 r) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, 64);
-        s1 += __shfl_xor(s1, off, 64);
-        s2 += __shfl_xor(s2, off, 64);
+        r.s0 += __shfl_xor(r.s0, off, 64);
+        r.s1 += __shfl_xor(r.s1, off, 64);
+        r.s2 += __shfl_xor(r.s2, off, 64);
+        r.m3 = fmin(r.m3, __shfl_xor(r.m3, off, 64));
     }
+    return r;
+}
+
+__device__ __forceinline__ void put_red4(double* out, const Red4& r) {
+    out[0] = r.s0;
+    out[1] = r.s1;
+    out[2] = r.s2;
+    out[3] = r.m3;
+}
+
+template <int NT>
+__device__ __forceinline__ void block_red4(Red4 r, double* out) {
+    __shared__ double sm[4][NT / 64];
+    r = wave_red4(r);
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
     if ((t & 63) == 0) {
-        sm[0][t >> 6] = s0;
-        sm[1][t >> 6] = s1;
-        sm[2][t >> 6] = s2;
+        sm[0][t >> 6] = r.s0;
+        sm[1][t >> 6] = r.s1;
+        sm[2][t >> 6] = r.s2;
+        sm[3][t >> 6] = r.m3;
     }
     __syncthreads();
     if (t == 0) {
-        double a0 = 0, a1 = 0, a2 = 0;
+        Red4 a = red4_init();
         for (int w = 0; w < NT / 64; ++w) {
-            a0 += sm[0][w];
-            a1 += sm[1][w];
-            a2 += sm[2][w];
+            a.s0 += sm[0][w];
+            a.s1 += sm[1][w];
+            a.s2 += sm[2][w];
+            a.m3 = fmin(a.m3, sm[3][w]);
         }
-        out[0] = a0;
-        out[1] = a1;
-        out[2] = a2;
+        put_red4(out, a);
     }
 }
 
-__device__ __forceinline__ void wave_sum3(double s0, double s1, double s2, double* out, int lane) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, 64);
-        s1 += __shfl_xor(s1, off, 64);
-        s2 += __shfl_xor(s2, off, 64);
-    }
-    if (lane == 0) {
-        out[0] = s0;
-        out[1] = s1;
-        out[2] = s2;
-    }
+// Health terms of one fluid node: rho and psi into the sums, a non-finite
+// rho or psi into the count, tau_mix - 1/2 (FP32) into the minimum.  NaN
+// drops out of the minimum (fmin); the count catches it.
+__device__ __forceinline__ void red4_add(Red4& r, float rho, float psi, float tex) {
+    r.s0 += (double)rho;
+    r.s1 += (double)psi;
+    if (!isfinite(rho) || !isfinite(psi)) r.s2 += 1.0;
+    r.m3 = fmin(r.m3, (double)tex);
 }
 
 // Where a lane's column sits: inside the domain, on a ghost column of the
@@ -300,8 +317,7 @@ __device__ __forceinline__ unsigned flags_of(const KParams& k, const KBufs& b, c
 template <int MODE>
 __device__ __forceinline__ void finish_node(const KParams& k, const KBufs& b, rsrc_t src, rsrc_t dst,
                                             const Lane& ln, const PullOff& po, int y, bool on, const float n[9],
-                                            float f[9], const float gc[9], unsigned flc, double& s_rho,
-                                            double& s_psi, double& s_bad) {
+                                            float f[9], const float gc[9], unsigned flc, Red4& red) {
     const float pc = n[0];
     float dpx, dpy, lap;
     fdn::stencil(n, dpx, dpy, lap);
@@ -356,9 +372,7 @@ __device__ __forceinline__ void finish_node(const KParams& k, const KBufs& b, rs
             rho = m.rho;
             ux = m.ux;
             uy = m.uy;
-            s_rho += (double)rho;
-            s_psi += (double)pc;
-            if (!isfinite(rho) || !isfinite(pc)) s_bad += 1.0;
+            red4_add(red, rho, pc, fdn::tau_excess(rho, pc, k.c));
         }
         b.fields[o] = pc;
         b.fields[k.Q + o] = rho;
@@ -439,7 +453,7 @@ __device__ __forceinline__ PsiRow psi_row_of(float pcen, float halo, int lane) {
 // round-robin over the eight XCDs) gets a contiguous band of chunks:
 // neighbouring strips and the halo rows of neighbouring chunks then share
 // that XCD's L2.  The FP64 partial sums of MODE 1 are per wave, at
-// partials[3 * (WPB * L + wave)].
+// partials[4 * (WPB * L + wave)] (Red4).
 // XCD-banded logical index of this workgroup: workgroups are dealt
 // round-robin over the eight XCDs, so XCD j runs blocks j, j+8, ...; they
 // are renumbered to one contiguous range per XCD.
@@ -455,10 +469,10 @@ __device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, in
     const int wave = threadIdx.x >> 6;
     const int chunk = L / k.nsb;
     const int strip = (k.sb0 + L - chunk * k.nsb) * WPB + wave;
-    double* part = b.partials + 3 * (WPB * L + wave);
+    double* part = b.partials + 4 * (WPB * L + wave);
     const int x0 = strip * 64;
     if (x0 >= k.W) {
-        if (MODE == 1 && lane == 0) part[0] = part[1] = part[2] = 0.0;
+        if (MODE == 1 && lane == 0) put_red4(part, red4_init());
         return;
     }
     const int xcol = x0 + lane;
@@ -472,7 +486,7 @@ __device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, in
     const unsigned bytes = 18u * (unsigned)k.Q * 4u;
     const rsrc_t src = make_rsrc(b.src, bytes);
     const rsrc_t dst = make_rsrc(b.dst, b.dst ? bytes : 0u);
-    double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
+    Red4 red = red4_init();
     const PullOff po = pull_offsets(k, ln.x);
 
     // Even chunks march up, odd chunks down: the two chunks beside a chunk
@@ -521,8 +535,7 @@ __device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, in
         const PsiRow& pn = (d > 0) ? pa : pb;  // row y + 1
         const PsiRow& ps = (d > 0) ? pb : pa;  // row y - 1
         const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, ps.c, pn.r, pn.l, ps.l, ps.r};
-        finish_node<MODE>(k, b, src, dst, ln, po, y, interior, nb9, f, gc, interior ? flc : kSolid, s_rho, s_psi,
-                          s_bad);
+        finish_node<MODE>(k, b, src, dst, ln, po, y, interior, nb9, f, gc, interior ? flc : kSolid, red);
         pb = pc;
         pc = pa;
 #pragma unroll
@@ -531,7 +544,10 @@ __device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, in
         fla = fl2;
         flah = fl2h;
     }
-    if (MODE == 1) wave_sum3(s_rho, s_psi, s_bad, part, lane);
+    if (MODE == 1) {
+        red = wave_red4(red);
+        if (lane == 0) put_red4(part, red);
+    }
 }
 
 template <int WPB, int MODE>
@@ -594,7 +610,7 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
                                                    const float* __restrict__ uy0 = nullptr) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
-    double s_rho = 0.0, s_psi = 0.0, s_bad = 0.0;
+    Red4 red = red4_init();
     if (x < k.W && y < k.H) {
         auto P = [&](int yy, int xx) -> float {
             if (k.y_walls && (yy < 0 || yy >= k.H)) return k.c.psi_wall;
@@ -650,11 +666,7 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             b.fields[k.Q + o] = solid ? 0.f : m.rho;
             b.fields[2 * k.Q + o] = solid ? 0.f : m.ux;
             b.fields[3 * k.Q + o] = solid ? 0.f : m.uy;
-            if (!solid) {
-                s_rho = (double)m.rho;
-                s_psi = (double)psi;
-                if (!isfinite(m.rho) || !isfinite(psi)) s_bad = 1.0;
-            }
+            if (!solid) red4_add(red, m.rho, psi, fdn::tau_excess(m.rho, psi, k.c));
         } else if (!k.x_periodic && (x == 0 || x == k.W - 1)) {
             const float mu = fdn::chem_pot(psi, lap, k.c);
             const int s = (x == 0) ? 0 : 2;
@@ -662,7 +674,95 @@ __global__ __launch_bounds__(256) void init_kernel(KParams k, KBufs b, const flo
             b.side_out[(s + 1) * k.H + y] = dpy * mu;
         }
     }
-    if (PRE) block_sum3<256>(s_rho, s_psi, s_bad, b.partials + 3 * (blockIdx.y * gridDim.x + blockIdx.x));
+    if (PRE) block_red4<256>(red, b.partials + 4 * (blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+// State from given post-stream populations of iteration N (what
+// fd_get_populations returns: after bounce-back, walls and Zou-He), one
+// thread per node.  `staged` holds them as 18 planes [H][Wp].
+//   COLLIDE = 0: fields (psi, rho, ux, uy) and FP64 partial sums of
+//                iteration N into b.fields / b.partials.
+//   COLLIDE = 1: the collision of iteration N into b.dst (the post-collision
+//                state the next launch pulls from) and P = d psi * mu of
+//                columns 0 and W-1 into b.side_out; on inert solid nodes the
+//                staged populations are cleared (nothing reads them).
+// psi of a neighbour is the sum of its staged g - the value the step kernel
+// forms after its own pull - so the collision equals the step kernel's bit
+// for bit.
+template <int COLLIDE>
+__global__ __launch_bounds__(256) void restage_kernel(KParams k, KBufs b, float* __restrict__ staged) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    Red4 red = red4_init();
+    if (x < k.W && y < k.H) {
+        auto P = [&](int yy, int xx) -> float {
+            if (k.y_walls && (yy < 0 || yy >= k.H)) return k.c.psi_wall;
+            yy = wrap(yy, k.H);
+            if (xx < 0 || xx >= k.W) {
+                if (!k.x_periodic) return xx < 0 ? 1.0f : -1.0f;
+                xx = wrap(xx, k.W);
+            }
+            const int o = yy * k.Wp + xx;
+            if (b.flags[o] & kSolid) return k.c.psi_wall;
+            const float* gp = staged + 9 * (size_t)k.Q + o;
+            return gp[0] + gp[k.Q] + gp[2 * k.Q] + gp[3 * k.Q] + gp[4 * k.Q] + gp[5 * k.Q] + gp[6 * k.Q] +
+                   gp[7 * k.Q] + gp[8 * k.Q];
+        };
+        const float n[9] = {P(y, x),         P(y, x + 1),     P(y + 1, x),
+                            P(y, x - 1),     P(y - 1, x),     P(y + 1, x + 1),
+                            P(y + 1, x - 1), P(y - 1, x - 1), P(y - 1, x + 1)};
+        float dpx, dpy, lap;
+        fdn::stencil(n, dpx, dpy, lap);
+        const float psi = n[0];
+        const int o = y * k.Wp + x;
+        const bool solid = (b.flags[o] & kSolid) != 0;
+        float f[9], g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            f[i] = staged[i * (size_t)k.Q + o];
+            g[i] = staged[(9 + i) * (size_t)k.Q + o];
+        }
+        if (COLLIDE) {
+            if (solid) {
+                // never collided: store-and-forward (or, inert, nothing)
+                if (k.inert) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) {
+                        f[i] = g[i] = 0.f;
+                        staged[i * (size_t)k.Q + o] = 0.f;
+                        staged[(9 + i) * (size_t)k.Q + o] = 0.f;
+                    }
+                }
+            } else {
+                const fdn::Mom<float> m = fdn::moments(f, psi, dpx, dpy, lap, k.c);
+                fdn::collide(f, g, m, k.c);
+                if (!k.x_periodic && (x == 0 || x == k.W - 1)) {
+                    const int s = (x == 0) ? 0 : 2;
+                    b.side_out[s * k.H + y] = dpx * m.mu;
+                    b.side_out[(s + 1) * k.H + y] = dpy * m.mu;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                b.dst[i * (size_t)k.Q + o] = f[i];
+                b.dst[(9 + i) * (size_t)k.Q + o] = g[i];
+            }
+        } else {
+            float rho = 0.f, ux = 0.f, uy = 0.f;
+            if (!solid) {
+                const fdn::Mom<float> m = fdn::moments(f, psi, dpx, dpy, lap, k.c);
+                rho = m.rho;
+                ux = m.ux;
+                uy = m.uy;
+                red4_add(red, rho, psi, fdn::tau_excess(rho, psi, k.c));
+            }
+            b.fields[o] = psi;
+            b.fields[k.Q + o] = rho;
+            b.fields[2 * k.Q + o] = ux;
+            b.fields[3 * k.Q + o] = uy;
+        }
+    }
+    if (!COLLIDE) block_red4<256>(red, b.partials + 4 * (blockIdx.y * gridDim.x + blockIdx.x));
 }
 
 // Streaming copy, the HBM yardstick: grid-stride 16-byte non-temporal
@@ -679,13 +779,14 @@ __global__ __launch_bounds__(256) void fd_copy_kernel(const float4* __restrict__
 
 __global__ __launch_bounds__(256) void reduce_partials(const double* __restrict__ part, int n,
                                                        double* __restrict__ out) {
-    double s0 = 0, s1 = 0, s2 = 0;
+    Red4 r = red4_init();
     for (int i = threadIdx.x; i < n; i += 256) {
-        s0 += part[3 * i];
-        s1 += part[3 * i + 1];
-        s2 += part[3 * i + 2];
+        r.s0 += part[4 * i];
+        r.s1 += part[4 * i + 1];
+        r.s2 += part[4 * i + 2];
+        r.m3 = fmin(r.m3, part[4 * i + 3]);
     }
-    block_sum3<256>(s0, s1, s2, out);
+    block_red4<256>(r, out);
 }
 
 // ------------------------------------------------------------------ host
@@ -704,6 +805,9 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 }  // namespace
+
+constexpr int kStageInit = 1;
+constexpr int kStagePops = 2;
 
 struct fd_handle {
     fd_params p{};
@@ -728,11 +832,21 @@ struct fd_handle {
     int cur = 0, sc = 0;
     int64_t N = 0;
     bool inited = false;
+    // What the buffers hold:
+    //   0          pop[cur] = post-collision state of iteration N-1 (N >= 1)
+    //   kStageInit fd_init / fd_init_fields: pop[cur] = post-collision state
+    //              of iteration N (= 0); the fields of N come from the staged
+    //              psi0/rho0 (and velocity)
+    //   kStagePops fd_set_populations: pop[cur] = post-collision state of
+    //              iteration N, pop[1-cur] = the given post-stream
+    //              populations of N, which the reads of N return
+    int stage = 0;
     bool init_u = false;  // started by fd_init_fields*: the velocity is staged in fields planes 2, 3
     int wx = 4;     // waves (64-column strips) per workgroup
     int rows = 16;  // rows per wave
     int timed_launches = 0;
     bool have_timing = false;
+    int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
 };
 
 namespace {
@@ -890,6 +1004,14 @@ int launch_step(fd_handle* h, int mode, const KBufs& b, int sb_lo = 0, int sb_hi
     return FD_OK;
 }
 
+// The iteration whose post-collision state pop[cur] holds.
+int64_t launched_through(const fd_handle* h) { return h->stage ? h->N : h->N - 1; }
+
+// Launches that take the handle from iteration N to N + n.
+int64_t launches_for(const fd_handle* h, int64_t n) {
+    return n > 0 ? h->N + n - 1 - launched_through(h) : 0;
+}
+
 // Number of FP64 partial-sum slots (one per wave) a moments launch writes.
 int moments_grid_blocks(const fd_handle* h) { return h->wx * step_grid(h, nullptr); }
 
@@ -898,13 +1020,15 @@ int ensure_partials(fd_handle* h, int n) {
     if (h->partials) FD_HIP(hipFree(h->partials));
     h->partials = nullptr;
     h->npart = 0;  // a failed allocation below must not leave a stale capacity behind
-    FD_HIP(hipMalloc(&h->partials, sizeof(double) * 3 * (size_t)n));
+    FD_HIP(hipMalloc(&h->partials, sizeof(double) * 4 * (size_t)n));
     h->npart = n;
     return FD_OK;
 }
 
 // Fields (and optionally post-stream populations into `pops`) of the current
-// iteration, plus the FP64 sums in h->mass.
+// iteration, plus the FP64 sums in h->mass.  With populations staged by
+// fd_set_populations the post-stream populations already sit in
+// pop[1 - cur]: `pops` may then only be that buffer (or NULL).
 int compute_fields(fd_handle* h, float* pops) {
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
     KBufs b{};
@@ -913,9 +1037,9 @@ int compute_fields(fd_handle* h, float* pops) {
     b.fields = h->fields;
     b.dst = pops;
     int nblk;
-    if (h->N == 0) {
-        const dim3 grid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
-        nblk = grid.x * grid.y;
+    const dim3 ngrid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
+    if (h->stage == kStageInit) {
+        nblk = ngrid.x * ngrid.y;
         int rc = ensure_partials(h, nblk);
         if (rc) return rc;
         b.partials = h->partials;
@@ -925,8 +1049,16 @@ int compute_fields(fd_handle* h, float* pops) {
         // writing the same one back
         const float* ux0 = h->init_u ? h->fields + 2 * h->Q : nullptr;
         const float* uy0 = h->init_u ? h->fields + 3 * h->Q : nullptr;
-        hipLaunchKernelGGL((init_kernel<1>), grid, dim3(64, 4), 0, h->stream, h->k, b, h->psi0, h->rho0,
+        hipLaunchKernelGGL((init_kernel<1>), ngrid, dim3(64, 4), 0, h->stream, h->k, b, h->psi0, h->rho0,
                            h->p.init_mode == FD_INIT_WALLED ? 1 : 0, ux0, uy0);
+        FD_HIP(hipGetLastError());
+    } else if (h->stage == kStagePops) {
+        if (pops && pops != h->pop[1 - h->cur]) return fail(FD_E_STATE, "internal: staged populations elsewhere");
+        nblk = ngrid.x * ngrid.y;
+        int rc = ensure_partials(h, nblk);
+        if (rc) return rc;
+        b.partials = h->partials;
+        hipLaunchKernelGGL((restage_kernel<0>), ngrid, dim3(64, 4), 0, h->stream, h->k, b, h->pop[1 - h->cur]);
         FD_HIP(hipGetLastError());
     } else {
         nblk = moments_grid_blocks(h);
@@ -947,7 +1079,7 @@ int compute_fields(fd_handle* h, float* pops) {
 // hold them; no FP64 sums).  At N = 0 the full initial fields are used.
 int compute_fields_window(fd_handle* h, int x0, int x1) {
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
-    if (h->N == 0) return compute_fields(h, nullptr);
+    if (h->stage) return compute_fields(h, nullptr);
     const int sb_lo = (x0 / 64) / h->wx, sb_hi = ((x1 - 1) / 64) / h->wx + 1;
     int rc = ensure_partials(h, h->wx * step_grid(h, nullptr, sb_lo, sb_hi));
     if (rc) return rc;
@@ -975,6 +1107,11 @@ const char* fd_last_error(void) { return g_err.c_str(); }
 
 void fd_destroy(fd_handle* h) {
     if (!h) return;
+    if (h->ens_refs > 0) {
+        // an ensemble's device table still points at this handle's buffers
+        g_err = "fd_destroy: the handle belongs to an ensemble; destroy the ensemble first (handle kept)";
+        return;
+    }
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (int i = 0; i < 2; ++i) {
@@ -1057,7 +1194,7 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     FD_HIP_H(hipMalloc(&h->psi0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->rho0, sizeof(float) * Q));
     FD_HIP_H(hipMalloc(&h->fields, sizeof(float) * 4 * Q));
-    FD_HIP_H(hipMalloc(&h->mass, sizeof(double) * 3));
+    FD_HIP_H(hipMalloc(&h->mass, sizeof(double) * 4));
     FD_HIP_H(hipMemcpyAsync(h->flags, fl.data(), sizeof(uint16_t) * Q, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipMemcpyAsync(h->uprof, up.data(), sizeof(float) * H, hipMemcpyHostToDevice, h->stream));
     FD_HIP_H(hipStreamSynchronize(h->stream));
@@ -1109,6 +1246,7 @@ int init_impl(fd_handle* h, const double* psi0, const double* rho0, const double
     h->sc = 0;
     h->N = 0;
     h->inited = true;
+    h->stage = kStageInit;
     h->init_u = ux0 != nullptr;
     h->have_timing = false;
     return FD_OK;
@@ -1130,11 +1268,10 @@ int fd_step(fd_handle* h, int32_t nsteps) {
     if (nsteps < 0) return fail(FD_E_ARG, "nsteps < 0");
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
     FD_HIP(hipSetDevice(h->device));
-    // The buffers hold the post-collision state of iteration max(N-1, 0);
-    // reaching iteration N' needs launches up to N'-1.
-    const int64_t have = std::max<int64_t>(h->N - 1, 0);
-    const int64_t want = std::max<int64_t>(h->N + nsteps - 1, 0);
-    const int64_t nl = want - have;
+    // The buffers hold the post-collision state of iteration
+    // launched_through(h); reaching iteration N' needs launches up to N'-1.
+    const int64_t have = launched_through(h);
+    const int64_t nl = launches_for(h, nsteps);
     if (nl > 0) FD_HIP(hipEventRecord(h->ev0, h->stream));
     for (int64_t i = 0; i < nl; ++i) {
         KBufs b{};
@@ -1145,7 +1282,16 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         b.side_out = h->side[1 - h->sc];
         b.uprof = h->uprof;
         int rc = launch_step(h, 0, b);
-        if (rc) return rc;
+        if (rc) {
+            // the i launches already issued stand: the buffers hold the
+            // post-collision state of iteration have + i
+            if (i > 0) {
+                h->N = have + i + 1;
+                h->stage = 0;
+            }
+            h->have_timing = false;
+            return rc;
+        }
         h->cur ^= 1;
         h->sc ^= 1;
     }
@@ -1155,6 +1301,7 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         h->have_timing = true;
     }
     h->N += nsteps;
+    if (nsteps > 0) h->stage = 0;
     return FD_OK;
 }
 
@@ -1289,6 +1436,14 @@ int fd_phase_mass(fd_handle* h, double out[3]) {
 }
 
 // State blob: header (magic, H, W, N) + side[4H] + 18 planes [H][W].
+// kMagicState: the planes are the post-collision state of iteration N-1 and
+// the side buffer belongs to it.  kMagicStaged: the planes are the
+// post-stream populations of iteration N (a state set by
+// fd_set_populations and not stepped since); the side buffer is unused and
+// loading it stages the populations again.
+constexpr int64_t kMagicState = 0x4644424d53544154LL;
+constexpr int64_t kMagicStaged = 0x4644424d53544147LL;
+
 int64_t fd_state_bytes(const fd_handle* h) {
     if (!h) return -1;
     return 32 + (int64_t)sizeof(float) * (4 * (int64_t)h->p.H + 18 * (int64_t)h->p.H * h->p.W);
@@ -1297,18 +1452,73 @@ int64_t fd_state_bytes(const fd_handle* h) {
 int fd_save_state(fd_handle* h, void* buf, int64_t nbytes) {
     if (!h || !buf) return fail(FD_E_ARG, "handle/buf is NULL");
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
+    if (h->stage == kStageInit)
+        return fail(FD_E_STATE, "nothing to save before the first iteration: fd_init rebuilds that state");
     if (nbytes < fd_state_bytes(h)) return fail(FD_E_ARG, "buffer too small");
     FD_HIP(hipSetDevice(h->device));
-    int64_t hdr[4] = {0x4644424d53544154LL, h->p.H, h->p.W, h->N};
+    const bool staged = h->stage == kStagePops;
+    int64_t hdr[4] = {staged ? kMagicStaged : kMagicState, h->p.H, h->p.W, h->N};
     std::memcpy(buf, hdr, 32);
     char* p = (char*)buf + 32;
     FD_HIP(hipMemcpyAsync(p, h->side[h->sc], sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToHost, h->stream));
     p += sizeof(float) * 4 * h->p.H;
+    const float* planes = staged ? h->pop[1 - h->cur] : h->pop[h->cur];
     for (int i = 0; i < 18; ++i) {
-        int rc = copy_plane_out(h, (float*)p, h->pop[h->cur] + i * h->Q);
+        int rc = copy_plane_out(h, (float*)p, planes + i * h->Q);
         if (rc) return rc;
         p += sizeof(float) * (size_t)h->p.H * h->p.W;
     }
+    FD_HIP(hipStreamSynchronize(h->stream));
+    return FD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Upload post-stream populations of iteration N ([9][H][W] f, then g; host
+// or, with kind hipMemcpyDeviceToDevice, device memory) into pop[1 - cur] and
+// run the collision of iteration N from them.
+int stage_populations(fd_handle* h, const float* f, const float* g, int64_t N, hipMemcpyKind kind) {
+    float* st = h->pop[1 - h->cur];
+    const size_t plane = (size_t)h->p.H * h->p.W;
+    for (int i = 0; i < 18; ++i) {
+        const float* src = i < 9 ? f + i * plane : g + (i - 9) * plane;
+        FD_HIP(hipMemcpy2DAsync(st + i * h->Q, sizeof(float) * h->Wp, src, sizeof(float) * h->p.W,
+                                sizeof(float) * h->p.W, h->p.H, kind, h->stream));
+    }
+    KBufs b{};
+    b.flags = h->flags;
+    b.dst = h->pop[h->cur];
+    b.side_out = h->side[h->sc];
+    const dim3 grid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
+    hipLaunchKernelGGL((restage_kernel<1>), grid, dim3(64, 4), 0, h->stream, h->k, b, st);
+    FD_HIP(hipGetLastError());
+    h->N = N;
+    h->inited = true;
+    h->stage = kStagePops;
+    h->init_u = false;
+    h->have_timing = false;
+    return FD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fd_set_populations(fd_handle* h, const float* f, const float* g) {
+    if (!h || !f || !g) return fail(FD_E_ARG, "handle/f/g is NULL");
+    FD_HIP(hipSetDevice(h->device));
+    int rc = stage_populations(h, f, g, h->inited ? h->N : 0, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    FD_HIP(hipStreamSynchronize(h->stream));  // the caller's host arrays may go away
+    return FD_OK;
+}
+
+int fd_watchdog(fd_handle* h, double out[2]) {
+    if (!h || !out) return fail(FD_E_ARG, "handle/out is NULL");
+    FD_HIP(hipSetDevice(h->device));
+    int rc = compute_fields(h, nullptr);
+    if (rc) return rc;
+    FD_HIP(hipMemcpyAsync(out, h->mass + 2, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
     FD_HIP(hipStreamSynchronize(h->stream));
     return FD_OK;
 }
@@ -1318,10 +1528,19 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
     if (nbytes < fd_state_bytes(h)) return fail(FD_E_ARG, "buffer too small");
     int64_t hdr[4];
     std::memcpy(hdr, buf, 32);
-    if (hdr[0] != 0x4644424d53544154LL || hdr[1] != h->p.H || hdr[2] != h->p.W || hdr[3] < 1)
-        return fail(FD_E_ARG, "state blob does not match this handle (or was saved at step 0)");
+    const bool staged = hdr[0] == kMagicStaged;
+    if ((hdr[0] != kMagicState && !staged) || hdr[1] != h->p.H || hdr[2] != h->p.W || hdr[3] < (staged ? 0 : 1))
+        return fail(FD_E_ARG, "state blob does not match this handle");
     FD_HIP(hipSetDevice(h->device));
     const char* p = (const char*)buf + 32;
+    if (staged) {
+        const float* planes = (const float*)(p + sizeof(float) * 4 * h->p.H);
+        const size_t plane = (size_t)h->p.H * h->p.W;
+        int rc = stage_populations(h, planes, planes + 9 * plane, hdr[3], hipMemcpyHostToDevice);
+        if (rc) return rc;
+        FD_HIP(hipStreamSynchronize(h->stream));
+        return FD_OK;
+    }
     FD_HIP(hipMemcpyAsync(h->side[h->sc], p, sizeof(float) * 4 * h->p.H, hipMemcpyHostToDevice, h->stream));
     p += sizeof(float) * 4 * h->p.H;
     for (int i = 0; i < 18; ++i) {
@@ -1332,6 +1551,8 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
     FD_HIP(hipStreamSynchronize(h->stream));
     h->N = hdr[3];
     h->inited = true;
+    h->stage = 0;
+    h->init_u = false;
     h->have_timing = false;
     return FD_OK;
 }
@@ -1405,6 +1626,7 @@ int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, cons
     h->cur = 0;
     h->sc = 0;
     h->N = 0;
+    h->stage = kStageInit;
     h->init_u = true;
     h->have_timing = false;
     return FD_OK;
@@ -1419,6 +1641,8 @@ int64_t fd_state_device_bytes(const fd_handle* h) {
 int fd_save_state_device(fd_handle* h, void* dbuf, int64_t nbytes) {
     if (!h || !dbuf) return fail(FD_E_ARG, "handle/buf is NULL");
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
+    if (h->stage) return fail(FD_E_STATE, "a device state blob is taken after at least one iteration since the last "
+                                          "fd_init / fd_set_populations");
     if (nbytes < fd_state_device_bytes(h)) return fail(FD_E_ARG, "buffer too small");
     FD_HIP(hipSetDevice(h->device));
     int rc = check_device_ptrs(h, {dbuf});
@@ -1443,6 +1667,8 @@ int fd_load_state_device(fd_handle* h, const void* dbuf, int64_t nbytes, int64_t
                           hipMemcpyDeviceToDevice, h->stream));
     h->N = steps_done;
     h->inited = true;
+    h->stage = 0;
+    h->init_u = false;
     h->have_timing = false;
     return FD_OK;
 }
@@ -1466,6 +1692,7 @@ struct fd_ensemble {
     hipGraphExec_t exec = nullptr;
     int timed_launches = 0;
     bool have_timing = false;
+    bool counted = false;            // holds a membership count on every handle
 };
 
 namespace {
@@ -1546,12 +1773,14 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
     if (!e) return fail(FD_E_ARG, "ensemble is NULL");
     if (nsteps < 0) return fail(FD_E_ARG, "nsteps < 0");
     const int64_t N = e->hs[0]->N;
+    const int64_t have = launched_through(e->hs[0]);
     for (fd_handle* h : e->hs) {
         if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called on every handle");
-        if (h->N != N) return fail(FD_E_STATE, "the handles of an ensemble must be at the same iteration");
+        if (h->N != N || launched_through(h) != have)
+            return fail(FD_E_STATE, "the handles of an ensemble must be at the same iteration and stage");
     }
     FD_HIP(hipSetDevice(e->device));
-    const int64_t nl = std::max<int64_t>(N + nsteps - 1, 0) - std::max<int64_t>(N - 1, 0);
+    const int64_t nl = launches_for(e->hs[0], nsteps);
     if (nl > 0) {
         int rc = FD_OK;
         bool consistent = e->table_ok;
@@ -1613,6 +1842,7 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
     }
     for (fd_handle* h : e->hs) {
         h->N += nsteps;
+        if (nsteps > 0) h->stage = 0;
         h->have_timing = false;
     }
     return FD_OK;
@@ -1653,6 +1883,8 @@ int fd_ens_create(fd_handle* const* hs, int32_t n, fd_ensemble** out) {
     FD_HIP_E(hipEventCreateWithFlags(&e->evd, hipEventDisableTiming));
     FD_HIP_E(hipMalloc(&e->table, sizeof(EnsDomain) * (size_t)n));
 #undef FD_HIP_E
+    for (fd_handle* h : e->hs) ++h->ens_refs;
+    e->counted = true;
     *out = e;
     return FD_OK;
 }
@@ -1695,6 +1927,8 @@ double fd_ens_last_step_ms(const fd_ensemble* e) {
 
 void fd_ens_destroy(fd_ensemble* e) {
     if (!e) return;
+    if (e->counted)
+        for (fd_handle* h : e->hs) --h->ens_refs;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->exec) (void)hipGraphExecDestroy(e->exec);

@@ -91,6 +91,16 @@ FD_HD T inv_tau_mix(T rho, T psi, const Consts<T>& c) {
     return rcp(tau);
 }
 
+// tau_mix - 1/2 = 6 M eta / (rho ((1-psi) + M (1+psi))), with the same psi
+// bound as inv_tau_mix: the health term of the watchdog.  BGK needs
+// tau_mix > 1/2; a value <= 0 (or infinite) means psi has crossed the pole
+// of the interpolation at psi = -(1+M)/(M-1) (fingering_periodic.py:201-208).
+template <typename T>
+FD_HD T tau_excess(T rho, T psi, const Consts<T>& c) {
+    const T pv = psi < -c.psi_visc ? -c.psi_visc : (psi > c.psi_visc ? c.psi_visc : psi);
+    return T(6) * c.M * c.eta_n * rcp(rho * ((T(1) - pv) + c.M * (T(1) + pv)));
+}
+
 // Moments the collision of one node uses.
 template <typename T>
 struct Mom {

@@ -46,6 +46,10 @@ class Ensemble:
         _native.check(self._lib.fd_ens_create(arr, len(self.computes), _native.ctypes.byref(h)))
         self._h = h
         self.tile = (4, 16)
+        # the device table points at every member's buffers: a member may not
+        # be closed while the ensemble exists (Compute.close raises)
+        for c in self.computes:
+            c._ensembles += 1
 
     @property
     def nodes(self):
@@ -95,6 +99,8 @@ class Ensemble:
         if getattr(self, "_h", None):
             self._lib.fd_ens_destroy(self._h)
             self._h = None
+            for c in self.computes:
+                c._ensembles -= 1
 
     def __del__(self):
         try:

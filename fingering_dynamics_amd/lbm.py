@@ -51,6 +51,8 @@ class Compute:
     """
 
     def __init__(self, mask, links, params: LbmParams, psi0=None, rho0=None):
+        self._ensembles = 0
+        self._h = None
         self._lib = _native.load()
         self.params = params
         self.mask = np.ascontiguousarray(mask, dtype=bool)
@@ -79,6 +81,9 @@ class Compute:
     def init(self, psi0, rho0=None):
         psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
         rho0 = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
+        for name, a in (("psi0", psi0), ("rho0", rho0)):
+            if a is not None and a.shape != (self.H, self.W):
+                raise ValueError("%s must be [H][W] = (%d, %d), got %s" % (name, self.H, self.W, a.shape))
         _native.check(self._lib.fd_init(self._h, _native.ptr(psi0), _native.ptr(rho0)))
         self._cache = None
 
@@ -156,19 +161,46 @@ class Compute:
         _native.check(self._lib.fd_get_populations(self._h, _native.ptr(f), _native.ptr(g)))
         return f, g
 
+    def set_populations(self, f, g):
+        """Replace the state by post-stream populations f, g [9][H][W] of the
+        current iteration (fd_set_populations): the next iteration collides
+        exactly these.  Stored in FP32."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (f, g)]
+        for name, a in zip("fg", arrs):
+            if a.shape != (9, self.H, self.W):
+                raise ValueError("%s must be [9][H][W] = (9, %d, %d), got %s" % (name, self.H, self.W, a.shape))
+        _native.check(self._lib.fd_set_populations(self._h, _native.ptr(arrs[0]), _native.ptr(arrs[1])))
+        self._cache = None
+
     @property
     def f(self):
         return self.populations()[0].astype(np.float64)
 
+    @f.setter
+    def f(self, value):
+        # cm.f = ... as in the reference's in-place writes (fingering_periodic.py:459)
+        self.set_populations(value, self.populations()[1])
+
     @property
     def g(self):
         return self.populations()[1].astype(np.float64)
+
+    @g.setter
+    def g(self, value):
+        self.set_populations(self.populations()[0], value)
 
     def mass(self):
         """FP64 sums over fluid nodes: (sum rho, sum psi, non-finite count)."""
         out = np.zeros(3, dtype=np.float64)
         _native.check(self._lib.fd_phase_mass(self._h, _native.ptr(out)))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def watchdog(self):
+        """(non-finite fluid nodes, min over fluid nodes of tau_mix - 1/2),
+        FP64 (fd_watchdog).  Healthy: 0 and a positive minimum."""
+        out = np.zeros(2, dtype=np.float64)
+        _native.check(self._lib.fd_watchdog(self._h, _native.ptr(out)))
+        return int(out[0]), float(out[1])
 
     # --------------------------------------------------------- checkpoint
     def save_state(self):
@@ -235,6 +267,8 @@ class Compute:
             raise ValueError("expected a contiguous float32 [H][W] device tensor")
 
     def close(self):
+        if getattr(self, "_ensembles", 0):
+            raise RuntimeError("this solver belongs to an open Ensemble; close the ensemble first")
         if getattr(self, "_h", None):
             self._lib.fd_destroy(self._h)
             self._h = None

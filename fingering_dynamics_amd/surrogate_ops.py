@@ -63,12 +63,26 @@ def _nhwc(x, dtype):
     return x.shape
 
 
+def _same_device(*ts):
+    """Every tensor on one GPU: the kernels dereference the raw pointers on
+    that device, so a host tensor or one of another device must not reach
+    them."""
+    dev = ts[0].device
+    for t in ts:
+        if t is not None and (not t.is_cuda or t.device != dev):
+            raise ValueError("every tensor must be on the same GPU device (got %s and %s)" % (dev, t.device))
+
+
 def conv3x3(x, w, bias=None, relu=False, out_dtype=torch.bfloat16):
     """y = act(conv3x3(x, w) + bias); x [N,H,W,Cin] bf16, w [Cout,3,3,Cin] bf16."""
     N, H, W, Cin = _nhwc(x, torch.bfloat16)
     Cout = w.shape[0]
     if tuple(w.shape) != (Cout, 3, 3, Cin) or w.dtype != torch.bfloat16 or not w.is_contiguous():
         raise ValueError("w must be contiguous bf16 [Cout, 3, 3, Cin]")
+    if bias is not None and (tuple(bias.shape) != (Cout,) or bias.dtype != torch.float32
+                             or not bias.is_contiguous()):
+        raise ValueError("bias must be a contiguous float32 [Cout] tensor")
+    _same_device(x, w, bias)
     y = torch.empty((N, H, W, Cout), dtype=out_dtype, device=x.device)
     y32 = y if out_dtype == torch.float32 else None
     y16 = y if out_dtype == torch.bfloat16 else None
@@ -79,6 +93,9 @@ def conv3x3(x, w, bias=None, relu=False, out_dtype=torch.bfloat16):
 
 def flip_weights(w):
     """[Cin,3,3,Cout] weights of the transposed convolution."""
+    if w.dim() != 4 or w.shape[1:3] != (3, 3) or w.dtype != torch.bfloat16 or not w.is_contiguous():
+        raise ValueError("w must be contiguous bf16 [Cout, 3, 3, Cin]")
+    _same_device(w)
     Cout, _, _, Cin = w.shape
     wt = torch.empty((Cin, 3, 3, Cout), dtype=torch.bfloat16, device=w.device)
     _check(load().fds_flip_weights(_p(w), _p(wt), Cin, Cout, _stream()))
@@ -97,6 +114,9 @@ def conv3x3_bwd_data(dy, wt, out_dtype=torch.bfloat16, relu_y=None):
     output of a ReLU forward) masks dy on load."""
     N, H, W, Cout = _nhwc(dy, torch.bfloat16)
     Cin = wt.shape[0]
+    if tuple(wt.shape) != (Cin, 3, 3, Cout) or wt.dtype != torch.bfloat16 or not wt.is_contiguous():
+        raise ValueError("wt must be contiguous bf16 [Cin, 3, 3, Cout] (flip_weights of w)")
+    _same_device(dy, wt, relu_y)
     dx = torch.empty((N, H, W, Cin), dtype=out_dtype, device=dy.device)
     _check(load().fds_conv3x3_bwd_data(_p(dy), _p(wt), _p(_mask(relu_y, dy)),
                                        _p(dx if out_dtype == torch.float32 else None),
@@ -109,8 +129,13 @@ def conv3x3_bwd_weight(x, dy, with_bias=True, scratch=None, relu_y=None):
     """(dw [Cout,3,3,Cin] float32, dbias [Cout] float32 or None); relu_y
     masks dy on load as in conv3x3_bwd_data."""
     N, H, W, Cin = _nhwc(x, torch.bfloat16)
-    Cout = dy.shape[3]
     _nhwc(dy, torch.bfloat16)
+    Cout = dy.shape[3]
+    if tuple(dy.shape[:3]) != (N, H, W):
+        raise ValueError("dy must be [N, H, W, Cout] with the N, H, W of x")
+    if scratch is not None and (scratch.dtype != torch.float32 or not scratch.is_contiguous()):
+        raise ValueError("scratch must be a contiguous float32 tensor")
+    _same_device(x, dy, relu_y, scratch)
     n = int(load().fds_bwd_weight_scratch(N, H, W, Cin, Cout))
     if scratch is None or scratch.numel() < n:
         scratch = torch.empty(n, dtype=torch.float32, device=x.device)

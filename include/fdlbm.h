@@ -136,6 +136,18 @@ int fd_get_fields_window(fd_handle* h, int32_t x0, int32_t x1, float* psi, float
  * iteration (cm.f, cm.g).  Either pointer may be NULL.  Synchronises. */
 int fd_get_populations(fd_handle* h, float* f, float* g);
 
+/* Replace the state by given post-stream populations f, g float[9*H*W] of
+ * the current iteration (the quantity fd_get_populations returns: after
+ * streaming, bounce-back, walls and Zou-He).  Replaces the in-place writes
+ * of cm.f / cm.g by the reference's loop (fingering_periodic.py:459-460,
+ * 464-467).  The iteration count is kept (0 on a handle not yet
+ * initialised); the next iteration collides exactly these populations, with
+ * psi and its stencil taken from them and the Zou-He side values P of this
+ * psi.  Reads before the next fd_step return these populations and their
+ * fields.  With inert solid nodes (fd_solid_inert) the values given on solid
+ * nodes are dropped (read back as 0).  Synchronises. */
+int fd_set_populations(fd_handle* h, const float* f, const float* g);
+
 /* FP64 sums over fluid nodes of the last completed iteration:
  * out[0] = sum rho, out[1] = sum psi, out[2] = number of fluid nodes whose
  * rho or psi is not finite.  No reference counterpart: the reference's only
@@ -143,11 +155,24 @@ int fd_get_populations(fd_handle* h, float* f, float* g);
  * Synchronises. */
 int fd_phase_mass(fd_handle* h, double out[3]);
 
+/* Failure watchdog of the last completed iteration, FP64:
+ * out[0] = number of fluid nodes whose rho or psi is not finite,
+ * out[1] = min over fluid nodes of tau_mix - 1/2 (FP32 per node).  BGK needs
+ * tau_mix > 1/2: out[1] <= 0 (or a huge value turning negative) means psi
+ * has crossed the pole psi = -(1+M)/(M-1) of the viscosity interpolation
+ * (fingering_periodic.py:201-208, :205); +inf if there is no fluid node.
+ * No reference counterpart: the reference's only guard is
+ * np.seterr(all='raise') (fingering_periodic.py:497).  Synchronises. */
+int fd_watchdog(fd_handle* h, double out[2]);
+
 /* Opaque resumable state (post-collision populations, Zou-He side buffer,
- * step counter).  fd_state_bytes gives the size; fd_save_state copies it
+ * step counter; or, right after fd_set_populations, the given
+ * populations).  fd_state_bytes gives the size; fd_save_state copies it
  * out, fd_load_state copies it back into a handle created with the same
- * fd_params and masks.  No reference counterpart (the reference keeps no
- * checkpoints, fingering_periodic.py:461-462). */
+ * fd_params and masks.  There is nothing to save between fd_init and the
+ * first fd_step (FD_E_STATE): fd_init rebuilds that state.  No reference
+ * counterpart (the reference keeps no checkpoints,
+ * fingering_periodic.py:461-462). */
 int64_t fd_state_bytes(const fd_handle* h);
 int fd_save_state(fd_handle* h, void* buf, int64_t nbytes);
 int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes);
@@ -164,7 +189,9 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes);
  *   fd_state_device_bytes / fd_save_state_device / fd_load_state_device
  *                          the resumable state (populations and Zou-He side
  *                          buffer) as a device blob; the step count is kept
- *                          by the caller and handed back on load (>= 1). */
+ *                          by the caller and handed back on load (>= 1).
+ *                          Saving needs an iteration since the last
+ *                          fd_init* / fd_set_populations (FD_E_STATE). */
 void* fd_stream(const fd_handle* h);
 int fd_fields_device(fd_handle* h, float* psi, float* rho, float* ux, float* uy);
 int fd_init_fields_device(fd_handle* h, const float* psi, const float* rho, const float* ux, const float* uy);
@@ -204,7 +231,10 @@ int fd_solid_inert(const fd_handle* h);
  * reference counterpart (a measurement aid). */
 int fd_copy_probe(int64_t nbytes, int32_t reps, double* ms_per_copy);
 
-/* Release every device buffer of the handle.  NULL is ignored. */
+/* Release every device buffer of the handle.  NULL is ignored.  A handle
+ * that still belongs to an ensemble is kept (its buffers are in the
+ * ensemble's device table): the call then does nothing and fd_last_error()
+ * says so; destroy the ensemble first. */
 void fd_destroy(fd_handle* h);
 
 /* ---- Ensembles: many independent domains advanced together ------------
@@ -215,7 +245,7 @@ void fd_destroy(fd_handle* h);
  * step loop are replayed from a captured HIP graph.  Each handle keeps its
  * own state, and fd_get_fields / fd_phase_mass / fd_save_state ... on a
  * handle stay valid (they are ordered after the ensemble's work).  Destroy
- * the ensemble before its handles. */
+ * the ensemble before its handles: fd_destroy refuses a member handle. */
 typedef struct fd_ensemble fd_ensemble;
 
 /* Batch n distinct, initialised handles of one device. */

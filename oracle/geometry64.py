@@ -20,7 +20,7 @@ Pinning
 rectangles  closed form: cv2.rectangle + binary_fill_holes of an in-image
             rectangle is the filled inclusive box; getCorner returns its four
             corners.  The link unions are pinned by a hand-enumerated known
-            answer (a single 6x6 block) in tests/test_oracle_geometry.py.
+            answer (a single 6x6 block) in tests/test_geometry.py.
 circles     *parity unpinned* for the raster: ``circle_outline`` is this
             project's restatement of an integer midpoint circle standing in
             for cv2.circle (cv2 is absent and the reference holds no golden

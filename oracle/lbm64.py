@@ -477,6 +477,26 @@ class Lbm64:
             wall_rows="edge" if (p.wall_lo == 0 and wall_hi == p.H - 1) else "shipped",
             psi_visc_bound=getattr(p, "psi_visc_bound", 0.0))
 
+    def set_populations(self, f, g):
+        """Replace f and g (post-stream, [9][H][W]) and recompute the moments
+        from them: the reference's in-place writes of cm.f / cm.g
+        (fingering_periodic.py:459-460, 464-467) followed by its moment
+        getters (:470-479).  The next step collides these populations."""
+        self.f = np.array(f, dtype=np.float64, copy=True)
+        self.g = np.array(g, dtype=np.float64, copy=True)
+        self.moments()
+
+    def health(self):
+        """(number of fluid nodes whose rho or psi is not finite,
+        min over fluid nodes of mix_tau - 1/2): the failure watchdog.  BGK
+        needs mix_tau > 1/2 (fingering_periodic.py:201-208, :258-260); the
+        reference's only guard is np.seterr(all='raise') (:497)."""
+        bad = ~np.isfinite(self.rho) | ~np.isfinite(self.psi[self.mask])
+        with np.errstate(invalid="ignore"):
+            tex = np.asarray(self.mix_tau, dtype=np.float64) - 0.5
+        tex = tex[~np.isnan(tex)]
+        return int(bad.sum()), float(tex.min()) if tex.size else float("inf")
+
     # ---------------------------------------------------------- outputs
     def field(self, name):
         """Full [H][W] view of a compressed moment (0 on solid nodes)."""
