@@ -9,9 +9,12 @@ one iteration of the hot loop of lattice_boltzmann/fingering_periodic.py
 domain; the populations stay resident in HBM.
 
 Prints ONE JSON line.  ``value`` is MLUPS over all H*W nodes.  ``roofline``
-prices the step kernel at 144 algorithmic bytes per fluid-node update
+prices the dominant kernel at 144 algorithmic bytes per fluid-node update
 (9 f + 9 g FP32 read and written) against 8 TB/s, using the kernel's average
-launch time from HIP events on the solver's stream.  ``cpu_baseline`` times
+launch time from HIP events on the solver's stream.  By default one launch
+advances two iterations (the intermediate state stays in registers), so a
+launch carries 2 x 144 algorithmic bytes per fluid node while moving the
+populations through HBM once.  ``cpu_baseline`` times
 the FP64 numpy restatement (the oracle, one core) on a few iterations of the
 same workload.
 
@@ -78,10 +81,11 @@ def workload(h=H, w=W):
     return mask, bits, params, circles
 
 
-def traffic_from_profiles():
-    """HBM bytes per step launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE), as recorded in profiles/; None if not recorded."""
-    path = os.path.join(ROOT, "profiles", "step_kernel_4096x2048_hbm_traffic.txt")
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE), as recorded in profiles/; None if not
+    recorded."""
+    path = os.path.join(ROOT, "profiles", "%s_4096x2048_hbm_traffic.txt" % kernel)
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -182,7 +186,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="iterations of the CPU baseline sample (0 skips it)")
-    ap.add_argument("--tile", default="", help="launch shape WX,ROWS (default: the library's choice)")
+    ap.add_argument("--tile", default="", help="launch shape WX,ROWS of the one-iteration kernel")
+    ap.add_argument("--fusion", type=int, default=2, help="iterations per launch (1 or 2; default 2)")
+    ap.add_argument("--ftile", default="", help="launch shape WX,ROWS of the two-iteration kernel")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
@@ -214,6 +220,8 @@ def main():
     if args.tile:
         wx, rows = (int(v) for v in args.tile.split(","))
         cm.set_tile(wx, rows)
+    fwx, frows = (int(v) for v in args.ftile.split(",")) if args.ftile else (0, 0)
+    cm.set_fusion(args.fusion, fwx, frows)
     cm.step(args.warmup)
     cm.synchronize()
     t0 = time.perf_counter()
@@ -222,14 +230,20 @@ def main():
     dt = time.perf_counter() - t0
     kernel_ms = cm.last_step_ms()
     s_rho, s_psi, bad = cm.mass()
+    _, tau_margin = cm.watchdog()
     cm.close()
     _, copy_gbps = copy_probe()
 
     ms_per_step = dt * 1e3 / args.steps
     mlups = H * W * args.steps / dt / 1e6
-    algo_bytes = BYTES_PER_FLUID_UPDATE * n_fluid
-    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
-    traffic = traffic_from_profiles()
+    # the dominant kernel: fused_kernel (two iterations per launch) or
+    # step_kernel (one); kernel_ms is the device time per iteration
+    per_launch = 2 if args.fusion == 2 else 1
+    kernel = "fused_kernel" if per_launch == 2 else "step_kernel"
+    algo_bytes = per_launch * BYTES_PER_FLUID_UPDATE * n_fluid
+    launch_ms = kernel_ms * per_launch
+    achieved = algo_bytes / (launch_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    traffic = traffic_from_profiles(kernel)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1) if achieved else None,
@@ -237,11 +251,14 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
         "traffic": traffic,
-        "kernel_ms": round(kernel_ms, 5),
+        "kernel": kernel,
+        "iterations_per_launch": per_launch,
+        "kernel_ms": round(launch_ms, 5),
+        "kernel_ms_per_iteration": round(kernel_ms, 5),
         "algorithmic_bytes_per_launch": algo_bytes,
         "fluid_nodes": n_fluid,
         # measured HBM bytes (rocprofv3, profiles/) over this run's kernel time
-        "hbm_rate_GBps": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic and kernel_ms > 0 else None,
+        "hbm_rate_GBps": round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic and kernel_ms > 0 else None,
         # the best streaming copy this device reached in the same process
         "copy_GBps": round(copy_gbps, 1),
     }
@@ -267,7 +284,7 @@ def main():
             "psi_wall": params.psi_wall,
         },
         "mflups": round(n_fluid * args.steps / dt / 1e6, 1),
-        "mass_check": {"sum_rho": s_rho, "sum_psi": s_psi, "non_finite": bad},
+        "mass_check": {"sum_rho": s_rho, "sum_psi": s_psi, "non_finite": bad, "min_tau_minus_half": tau_margin},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -55,6 +55,7 @@ EXPORTS = {
     "fd_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_last_step_ms": (ctypes.c_double, [ctypes.c_void_p]),
     "fd_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "fd_set_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fd_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "fd_destroy": (None, [ctypes.c_void_p]),
     "fd_ens_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),

@@ -42,12 +42,17 @@ namespace {
 //              bottom walls (fingering.py:432-451, 573) are links of their
 //              rows, set by the host
 //   bit 8      solid
+//   bit 9      dead piece (see kDeadPiece)
 // In the inert-solid mode (leak-free geometry) solid nodes are never read
 // and store zeros (or nothing, in 64-byte groups of inert solids);
 // otherwise they stream their populations on unchanged (store-and-forward,
 // fingering_periodic.py:455-466).
 constexpr unsigned kLinks = 0xFFu;
 constexpr unsigned kSolid = 1u << 8;
+// bit 9: the node's aligned 16-column piece (64 B of a plane row) holds
+// only inert solid nodes (or columns beyond W): nothing ever reads it, so
+// no launch writes it.  Set by the host in the inert-solid mode.
+constexpr unsigned kDeadPiece = 1u << 9;
 // A vector offset beyond any descriptor's range: such buffer loads return 0
 // and such stores are dropped, without a memory access.
 constexpr unsigned kOOB = 0x7FFFFFFCu;
@@ -286,8 +291,8 @@ __device__ __forceinline__ Lane lane_of(const KParams& k, int x) {
     l.inside = true;
     l.ghost = 0.f;
     if (x < 0 || x >= k.W) {
-        if (k.x_periodic && x >= -1 && x <= k.W) {
-            x = (x < 0) ? x + k.W : x - k.W;
+        if (k.x_periodic) {
+            x = ((x % k.W) + k.W) % k.W;
         } else {
             // ghost columns +1 at x = -1 and -1 at x = W (fingering_periodic.py:94-95)
             l.inside = false;
@@ -553,6 +558,291 @@ __device__ __forceinline__ void march_block(const KParams& k, const KBufs& b, in
 template <int WPB, int MODE>
 __global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
     march_block<WPB, MODE>(k, b, banded_block());
+}
+
+// ---------------------------------------------------- two-iteration march
+// One launch advances the state by two iterations: S_n -> S_n+2, with
+// S_n+1 kept in registers and never written to HBM, so each iteration moves
+// half the bytes of a single-iteration launch.
+//
+// A wave owns a strip of kFuseCols output columns [xb0, xb0 + 60) and its
+// 64 lanes hold columns a0 = xb0 - 2 ... xb0 + 61.  Stage A is the
+// single-iteration march on all 64 lanes (S_n from HBM -> S_n+1 of the
+// strip, P_n+1 of the Zou-He columns); stage B marches two rows behind it
+// and pulls S_n+1 from A's last rows in registers: a population moving in
+// x comes from the neighbouring lane by a DPP lane shift (wave_shr:1 /
+// wave_shl:1).  B needs A's output two columns either side of a node (one
+// for the f pull, one more for the g pulls of the psi stencil), so it is
+// valid on lanes 2..61: strips overlap by four columns of stage-A work.
+// For the chunk of output rows [y0, y1) stage A runs rows y0-2 ... y1+1.
+//
+// Every node of either stage runs the single-iteration kernel's arithmetic
+// on the same values in the same order, so the result equals two
+// single-iteration launches bit for bit.
+constexpr int kFuseCols = 60;
+
+__device__ __forceinline__ float lane_from_left(float v) {  // lane l <- lane l-1 (DPP wave_shr:1)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_from_right(float v) {  // lane l <- lane l+1 (DPP wave_shl:1)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+// value v of the lane at column x - ex (ex in {-1, 0, 1})
+__device__ __forceinline__ float lane_pull(float v, int ex) {
+    return ex > 0 ? lane_from_left(v) : (ex < 0 ? lane_from_right(v) : v);
+}
+
+// Stage B's pull of one distribution for the node of row y at this lane
+// from stage A's post-collision rows y-1 (lo, only directions 2, 5, 6 are
+// read), y (mid) and y+1 (hi); a linked direction i takes the node's own
+// mid[opp(i)] (half-way bounce-back).  Lanes with !on get 0 (inert solids,
+// ghost columns), as the single-iteration kernel's unloaded lanes do.
+__device__ __forceinline__ void reg_pull(const float lo[9], const float mid[9], const float hi[9], unsigned links,
+                                         bool on, float v[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float* row = fdn::EY[i] > 0 ? lo : (fdn::EY[i] < 0 ? hi : mid);
+        float x = lane_pull(row[i], fdn::EX[i]);
+        if (i > 0) x = ((links >> (i - 1)) & 1u) ? mid[fdn::OPP[i]] : x;
+        v[i] = on ? x : 0.f;
+    }
+}
+
+// Inlet density (fingering_periodic.py:276-277) of the node of row y at
+// column 0 from stage A's rows y-1, y, y+1: the adjacent-row density of the
+// walled inlet corners (fingering.py:334-364).  Every lane executes it.
+__device__ __forceinline__ float reg_inlet_rho(const float lo[9], const float mid[9], const float hi[9],
+                                               unsigned links, float u, float px) {
+    float f[9];
+    reg_pull(lo, mid, hi, links, true, f);
+    return fdn::inlet_rho(f, u, px);
+}
+
+// Stage B's Zou-He / walled-corner update of f at node (y, x): as bc_f,
+// with the side values P (of stage A's psi) and the corner density given.
+__device__ __forceinline__ void reg_bc_f(const KParams& k, const KBufs& b, int y, int x, float f[9], float px,
+                                         float py, float rho_nb) {
+    if (k.x_periodic) return;
+    if (x == 0) {
+        const bool bot = k.y_walls && y == 0;
+        const bool top = k.y_walls && y == k.H - 1;
+        float n[9], cb[9], ct[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) n[i] = cb[i] = ct[i] = f[i];
+        fdn::inlet_f(n, b.uprof[y], px, py);
+        fdn::inlet_corner_bottom_f(cb, rho_nb);
+        fdn::inlet_corner_top_f(ct, rho_nb);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[i] = bot ? cb[i] : (top ? ct[i] : n[i]);
+    }
+    if (x == k.W - 1) {
+        fdn::outlet_f(f, b.uprof[y], px, py, k.pressure != 0, k.rho_out, k.c3);
+        const float f2 = (k.y_walls && y == 0) ? f[4] : f[2];
+        const float f4 = (k.y_walls && y == k.H - 1) ? f[2] : f[4];
+        f[2] = f2;
+        f[4] = f4;
+    }
+}
+
+// Moments and collision of one node (identity on solid lanes), as in
+// finish_node MODE 0; returns mu for the side values.
+__device__ __forceinline__ float collide_node(const KParams& k, float f[9], float g[9], float pc, float dpx,
+                                              float dpy, float lap, bool solid) {
+    fdn::Mom<float> m = fdn::moments(f, pc, dpx, dpy, lap, k.c);
+    const float mu = m.mu;
+    fdn::Consts<float> cl = k.c;
+    if (solid) {
+        m.rho = m.psi = m.mu = m.ux = m.uy = m.p = m.itau = m.fx = m.fy = 0.f;
+        cl.inv_tau_g = 0.f;
+    }
+    fdn::collide(f, g, m, cl);
+    return mu;
+}
+
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
+    const int L = banded_block();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = L / k.nsb;
+    const int strip = (L - chunk * k.nsb) * WPB + wave;
+    const int xb0 = strip * kFuseCols;
+    if (xb0 >= k.W) return;
+    const int a0 = xb0 - 2;
+    const int xa = a0 + lane;
+    const Lane ln = lane_of(k, xa);
+    const Lane lh = lane_of(k, lane == 0 ? a0 - 1 : a0 + 64);
+    const bool hl = (lane == 0) || (lane == 63);
+    const bool bout = lane >= 2 && lane < 2 + kFuseCols && xa < k.W;   // stage B stores this lane
+    const bool zx = !k.x_periodic;
+    const bool has_in = zx && xb0 == 0;                                  // column 0 among the outputs
+    const int y0 = chunk * k.rows;
+    const int y1 = min(y0 + k.rows, k.H);
+
+    const unsigned bytes = 18u * (unsigned)k.Q * 4u;
+    const rsrc_t src = make_rsrc(b.src, bytes);
+    const rsrc_t dst = make_rsrc(b.dst, bytes);
+    const PullOff po = pull_offsets(k, ln.x);
+
+    // ---- stage A prologue: psi_n+1 of rows y0-3, y0-2 and g of row y0-2
+    const int ys = y0 - 2;
+    float gc[9];
+    PsiRow pb, pc;
+    {
+        const int yb = ys - 1;
+        const unsigned fb = flags_of(k, b, ln, yb), fbh = flags_of(k, b, lh, yb);
+        const bool on = g_on(k, ln, yb, fb);
+        pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, yb, fbh);
+        pb = psi_row_of(finish_g(k, ln, yb, fb, on, gc), h, lane);
+    }
+    unsigned flc = flags_of(k, b, ln, ys);
+    {
+        const unsigned fch = flags_of(k, b, lh, ys);
+        const bool on = g_on(k, ln, ys, flc);
+        pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, gc);
+        float h = 0.f;
+        if (hl) h = psi_halo(k, src, lh, ys, fch);
+        pc = psi_row_of(finish_g(k, ln, ys, flc, on, gc), h, lane);
+    }
+    unsigned fla = flags_of(k, b, ln, ys + 1);
+    unsigned flah = flags_of(k, b, lh, ys + 1);
+
+    // ---- the hand-over ring: A's post-collision rows r (Fa, Ga), r-1
+    // (Fb, Gb), r-2 (Fc; Gc only directions 2, 5, 6), r-3 (Fd: 2, 5, 6);
+    // A's flags of rows r, r-1, r-2; A's side values P of rows r, r-1, r-2
+    float Fa[9], Fb[9], Fc[9], Fd[9], Ga[9], Gb[9], Gc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fa[i] = Fb[i] = Fc[i] = Fd[i] = Ga[i] = Gb[i] = Gc[i] = 0.f;
+    unsigned fr0 = kSolid, fr1 = kSolid, fr2 = kSolid;
+    float pxa = 0.f, pya = 0.f, pxb = 0.f, pyb = 0.f, pxc = 0.f, pyc = 0.f;
+    // ---- stage B state: psi_n+2 of rows y-1 (qs), y (qc); g of row y (gq);
+    // inlet density of the previous row (walled inlet corners)
+    PsiRow qs{}, qc{};
+    float gq[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gq[i] = 0.f;
+    float rho_prev = 0.f;
+
+    const int nj = (y1 - y0) + 4;
+    for (int j = 0; j < nj; ++j) {
+        const int r = ys + j;        // stage A row
+        const bool arow = !(k.y_walls && (r < 0 || r >= k.H));
+        const int rw = wrap(r, k.H);
+        // ---------------- stage A, row r
+        float Fn[9], Gn[9], pxn = 0.f, pyn = 0.f;
+        {
+            const int ya = r + 1;
+            const unsigned fl2 = flags_of(k, b, ln, ya + 1);
+            const unsigned fl2h = flags_of(k, b, lh, ya + 1);
+            float ga[9], f[9];
+            const bool ga_on = g_on(k, ln, ya, fla);
+            pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
+            const bool f_on = arow && ln.inside && !((flc & kSolid) && k.inert);
+            pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, f_on, f);
+            float h = 0.f;
+            if (hl) h = psi_halo(k, src, lh, ya, flah);
+            const PsiRow pa = psi_row_of(finish_g(k, ln, ya, fla, ga_on, ga), h, lane);
+            const float nb9[9] = {pc.c, pc.r, pa.c, pc.l, pb.c, pa.r, pa.l, pb.l, pb.r};
+            float dpx, dpy, lap;
+            fdn::stencil(nb9, dpx, dpy, lap);
+            if (arow) bc_f(k, b, src, rw, (int)ln.x, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Gn[i] = gc[i];
+            const bool solid = !arow || !ln.inside || (flc & kSolid) != 0;
+            const float mu = collide_node(k, f, Gn, pc.c, dpx, dpy, lap, solid);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fn[i] = f[i];
+            // P = d psi * mu of this row's psi at the Zou-He columns (the
+            // value the next iteration's Zou-He reads)
+            pxn = dpx * mu;
+            pyn = dpy * mu;
+            pb = pc;
+            pc = pa;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) gc[i] = ga[i];
+            fr2 = fr1;
+            fr1 = fr0;
+            fr0 = arow && ln.inside ? flc : kSolid;
+            flc = fla;
+            fla = fl2;
+            flah = fl2h;
+        }
+        // ---------------- hand-over
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Fd[i] = Fc[i];
+            Fc[i] = Fb[i];
+            Fb[i] = Fa[i];
+            Fa[i] = Fn[i];
+            Gc[i] = Gb[i];
+            Gb[i] = Ga[i];
+            Ga[i] = Gn[i];
+        }
+        pxc = pxb;
+        pyc = pyb;
+        pxb = pxa;
+        pyb = pya;
+        pxa = pxn;
+        pya = pyn;
+        if (j < 2) continue;
+        // ---------------- stage B: psi_n+2 and g of row r-1
+        PsiRow qn;
+        float gn[9];
+        {
+            const int yq = r - 1;
+            const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr1 & kSolid) && k.inert);
+            reg_pull(Gc, Gb, Ga, fr1 & kLinks, on, gn);
+            const float q = finish_g(k, ln, yq, fr1, on, gn);
+            qn.c = q;
+            qn.l = lane_from_left(q);
+            qn.r = lane_from_right(q);
+        }
+        // walled inlet corners need the inlet density of the adjacent row:
+        // row y0-1 is the one stage B has not marched (top corner at y0 = H-1)
+        if (j == 3 && has_in && k.y_walls && y0 >= 1)
+            rho_prev = reg_inlet_rho(Fd, Fc, Fb, fr2 & kLinks, b.uprof[y0 - 1], pxc);
+        if (j >= 4) {
+            // ------------ stage B: collision of row y = r-2
+            const int y = r - 2;
+            const unsigned fl = fr2;
+            const bool solid = (fl & kSolid) != 0;
+            float f[9];
+            reg_pull(Fd, Fc, Fb, fl & kLinks, ln.inside && !(solid && k.inert), f);
+            float rho_nb = rho_prev;
+            if (has_in) {
+                if (k.y_walls && y == 0) rho_nb = reg_inlet_rho(Fc, Fb, Fa, fr1 & kLinks, b.uprof[1], pxb);
+                rho_prev = fdn::inlet_rho(f, b.uprof[y], pxc);
+            }
+            const float nb9[9] = {qc.c, qc.r, qn.c, qc.l, qs.c, qn.r, qn.l, qs.l, qs.r};
+            float dpx, dpy, lap;
+            fdn::stencil(nb9, dpx, dpy, lap);
+            reg_bc_f(k, b, y, (int)ln.x, f, pxc, pyc, rho_nb);
+            float g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] = gq[i];
+            const float mu = collide_node(k, f, g, qc.c, dpx, dpy, lap, solid || !ln.inside);
+            if (bout && zx && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
+                const int sd = (ln.x == 0) ? 0 : 2;
+                b.side_out[sd * k.H + y] = dpx * mu;
+                b.side_out[(sd + 1) * k.H + y] = dpy * mu;
+            }
+            const unsigned vo = (bout && !(fl & kDeadPiece)) ? ln.x * 4u : kOOB;
+            const int rc = y * k.Wp;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, (rc + i * k.Q) * 4,
+                                                      0);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo,
+                                                      (rc + (9 + i) * k.Q) * 4, 0);
+            }
+        }
+        qs = qc;
+        qc = qn;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gq[i] = gn[i];
+    }
 }
 
 // One domain of a batched launch (an ensemble): its constants, flags,
@@ -844,6 +1134,9 @@ struct fd_handle {
     bool init_u = false;  // started by fd_init_fields*: the velocity is staged in fields planes 2, 3
     int wx = 4;     // waves (64-column strips) per workgroup
     int rows = 16;  // rows per wave
+    int fuse = 2;   // iterations per launch of fd_step (1, or 2: fused_kernel)
+    int fwx = 1;    // fused_kernel: waves per workgroup
+    int frows = 24; // fused_kernel: output rows per wave
     int timed_launches = 0;
     bool have_timing = false;
     int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
@@ -936,6 +1229,14 @@ bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, 
             if (sol(y, x)) v = inert ? kSolid : (v | kSolid);
             fl[(size_t)y * Wp + x] = (uint16_t)v;
         }
+    if (inert)
+        for (int y = 0; y < H; ++y)
+            for (int x0 = 0; x0 < Wp; x0 += 16) {
+                bool dead = true;
+                for (int x = x0; x < x0 + 16 && dead; ++x) dead = fl[(size_t)y * Wp + x] == kSolid;
+                if (dead)
+                    for (int x = x0; x < x0 + 16; ++x) fl[(size_t)y * Wp + x] = (uint16_t)(kSolid | kDeadPiece);
+            }
     return inert;
 }
 
@@ -1010,6 +1311,35 @@ int64_t launched_through(const fd_handle* h) { return h->stage ? h->N : h->N - 1
 // Launches that take the handle from iteration N to N + n.
 int64_t launches_for(const fd_handle* h, int64_t n) {
     return n > 0 ? h->N + n - 1 - launched_through(h) : 0;
+}
+
+// Two iterations in one launch (fused_kernel): b.src holds S_n, b.dst
+// receives S_n+2; side_in P_n, side_out P_n+2.  Stage A treats rows beyond
+// a wall as ghost rows, which holds when the walls are the edge rows (every
+// population that would stream in across y = 0 / H-1 is bounced there).
+// With the walls of fingering.py:573 on rows 1 and H-2, rows 0 and H-1 pull
+// across the edge through np.roll's wrap: those domains run one launch per
+// iteration.
+bool fusable(const fd_handle* h) {
+    const bool edge_walls = h->p.y_bc != FD_Y_WALLS || (h->p.wall_lo == 0 && h->p.wall_hi == h->p.H - 1);
+    return h->fuse == 2 && h->p.H >= 8 && edge_walls;
+}
+
+int launch_fused(fd_handle* h, const KBufs& b) {
+    KParams k = h->k;
+    k.rows = h->frows;
+    const int nstrips = (h->p.W + kFuseCols - 1) / kFuseCols;
+    k.nsb = (nstrips + h->fwx - 1) / h->fwx;
+    k.sb0 = 0;
+    const int nchunks = (h->p.H + h->frows - 1) / h->frows;
+    const dim3 grid(k.nsb * nchunks), block(64 * h->fwx);
+    switch (h->fwx) {
+        case 1: hipLaunchKernelGGL((fused_kernel<1>), grid, block, 0, h->stream, k, b); break;
+        case 4: hipLaunchKernelGGL((fused_kernel<4>), grid, block, 0, h->stream, k, b); break;
+        default: hipLaunchKernelGGL((fused_kernel<2>), grid, block, 0, h->stream, k, b); break;
+    }
+    FD_HIP(hipGetLastError());
+    return FD_OK;
 }
 
 // Number of FP64 partial-sum slots (one per wave) a moments launch writes.
@@ -1273,7 +1603,7 @@ int fd_step(fd_handle* h, int32_t nsteps) {
     const int64_t have = launched_through(h);
     const int64_t nl = launches_for(h, nsteps);
     if (nl > 0) FD_HIP(hipEventRecord(h->ev0, h->stream));
-    for (int64_t i = 0; i < nl; ++i) {
+    for (int64_t i = 0; i < nl;) {
         KBufs b{};
         b.src = h->pop[h->cur];
         b.dst = h->pop[1 - h->cur];
@@ -1281,7 +1611,9 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         b.side_in = h->side[h->sc];
         b.side_out = h->side[1 - h->sc];
         b.uprof = h->uprof;
-        int rc = launch_step(h, 0, b);
+        // pairs of iterations go to the two-iteration kernel
+        const int it = (fusable(h) && nl - i >= 2) ? 2 : 1;
+        int rc = it == 2 ? launch_fused(h, b) : launch_step(h, 0, b);
         if (rc) {
             // the i launches already issued stand: the buffers hold the
             // post-collision state of iteration have + i
@@ -1294,6 +1626,7 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         }
         h->cur ^= 1;
         h->sc ^= 1;
+        i += it;
     }
     if (nl > 0) {
         FD_HIP(hipEventRecord(h->ev1, h->stream));
@@ -1369,6 +1702,17 @@ double fd_last_step_ms(const fd_handle* h) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return -1.0;
     return (double)ms / h->timed_launches;
+}
+
+int fd_set_fusion(fd_handle* h, int32_t depth, int32_t waves_x, int32_t rows) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    if (depth != 1 && depth != 2) return fail(FD_E_ARG, "depth must be 1 or 2");
+    if (waves_x > 0 && waves_x != 1 && waves_x != 2 && waves_x != 4)
+        return fail(FD_E_ARG, "waves_x must be 1, 2 or 4 (or <= 0 to keep)");
+    h->fuse = depth;
+    if (waves_x > 0) h->fwx = waves_x;
+    if (rows > 0) h->frows = rows;
+    return FD_OK;
 }
 
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows) {

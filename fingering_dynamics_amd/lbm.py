@@ -122,6 +122,13 @@ class Compute:
         marching `rows` rows of its own 64-column strip."""
         _native.check(self._lib.fd_set_tile(self._h, int(waves_x), int(rows)))
 
+    def set_fusion(self, depth, waves_x=0, rows=0):
+        """Iterations per launch: 2 (default) keeps the intermediate state of
+        each pair of iterations in registers; 1 runs one launch per
+        iteration.  Identical results.  waves_x / rows (> 0) set the
+        two-iteration kernel's launch shape."""
+        _native.check(self._lib.fd_set_fusion(self._h, int(depth), int(waves_x), int(rows)))
+
     # ------------------------------------------------------------ outputs
     def fields(self):
         """(psi, rho, ux, uy) as float32 [H][W] (rho/u are 0 on solid nodes)."""

@@ -212,6 +212,17 @@ double fd_last_step_ms(const fd_handle* h);
  * implementation). */
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
 
+/* Iterations per kernel launch of fd_step: depth 2 (the default) advances
+ * pairs of iterations with one launch that keeps the intermediate state in
+ * registers, so each iteration reads and writes HBM once per two
+ * iterations; depth 1 runs one launch per iteration.  The results are
+ * identical bit for bit.  waves_x (1, 2 or 4) and rows set the two-iteration
+ * kernel's launch shape (<= 0 keeps the current one; default 1 x 24).
+ * Domains with H < 8, and walled domains whose walls are not the edge rows
+ * (the rows 1 and H-2 of fingering.py:573), always run depth 1.  No reference counterpart (a
+ * launch control of this implementation). */
+int fd_set_fusion(fd_handle* h, int32_t depth, int32_t waves_x, int32_t rows);
+
 /* 1 if the handle runs with inert solid nodes, 0 if solid nodes stream
  * their populations on (store-and-forward, fingering_periodic.py:455-466),
  * -1 for NULL.  Inert needs a leak-free geometry: every solid -> fluid link

@@ -96,8 +96,8 @@ def test_oracle_mid_trajectory_state(case):
 
 def test_oracle_mid_trajectory_state_bench_workload():
     """The bench's boundary family (walls, pressure outlet, complete links,
-    calibrated psi_wall, M = 10) on a 128 x 256 copy, from iteration 60."""
-    mask, bits, params, _ = bench.workload(128, 256)
+    calibrated psi_wall, M = 10) on a 256 x 512 copy, from iteration 60."""
+    mask, bits, params, _ = bench.workload(256, 512)
     _oracle_state_pair(params, mask, bits, geometry.unpack_links(bits), 60, [1, 2, 5, 20])
 
 
