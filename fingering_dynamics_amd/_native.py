@@ -42,6 +42,7 @@ EXPORTS = {
     "fd_get_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_set_populations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_phase_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fd_phase_mass_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fd_watchdog": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fd_state_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),

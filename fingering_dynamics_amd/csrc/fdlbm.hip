@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1359,7 +1360,7 @@ int ensure_partials(fd_handle* h, int n) {
 // iteration, plus the FP64 sums in h->mass.  With populations staged by
 // fd_set_populations the post-stream populations already sit in
 // pop[1 - cur]: `pops` may then only be that buffer (or NULL).
-int compute_fields(fd_handle* h, float* pops) {
+int compute_fields(fd_handle* h, float* pops, double* scratch = nullptr, int scratch_slots = 0) {
     if (!h->inited) return fail(FD_E_STATE, "fd_init has not been called");
     KBufs b{};
     b.flags = h->flags;
@@ -1367,12 +1368,28 @@ int compute_fields(fd_handle* h, float* pops) {
     b.fields = h->fields;
     b.dst = pops;
     int nblk;
+    // FP64 partial sums: the handle's buffer (grown on demand), or a
+    // caller-provided fixed scratch area
+    double* part = nullptr;
+    auto take_partials = [&](int n) -> int {
+        if (scratch) {
+            if (n > scratch_slots)
+                return fail(FD_E_ARG, "this launch shape needs " + std::to_string(n) +
+                                          " partial-sum slots; the fixed scratch holds " +
+                                          std::to_string(scratch_slots));
+            part = scratch;
+            return FD_OK;
+        }
+        int rc = ensure_partials(h, n);
+        part = h->partials;
+        return rc;
+    };
     const dim3 ngrid((h->p.W + 63) / 64, (h->p.H + 3) / 4);
     if (h->stage == kStageInit) {
         nblk = ngrid.x * ngrid.y;
-        int rc = ensure_partials(h, nblk);
+        int rc = take_partials(nblk);
         if (rc) return rc;
-        b.partials = h->partials;
+        b.partials = part;
         if (!pops) b.dst = h->pop[1 - h->cur];
         // after fd_init_fields the staged velocity (fields planes 2, 3) is
         // part of the N = 0 state; each node reads its own value before
@@ -1385,22 +1402,22 @@ int compute_fields(fd_handle* h, float* pops) {
     } else if (h->stage == kStagePops) {
         if (pops && pops != h->pop[1 - h->cur]) return fail(FD_E_STATE, "internal: staged populations elsewhere");
         nblk = ngrid.x * ngrid.y;
-        int rc = ensure_partials(h, nblk);
+        int rc = take_partials(nblk);
         if (rc) return rc;
-        b.partials = h->partials;
+        b.partials = part;
         hipLaunchKernelGGL((restage_kernel<0>), ngrid, dim3(64, 4), 0, h->stream, h->k, b, h->pop[1 - h->cur]);
         FD_HIP(hipGetLastError());
     } else {
         nblk = moments_grid_blocks(h);
-        int rc = ensure_partials(h, nblk);
+        int rc = take_partials(nblk);
         if (rc) return rc;
-        b.partials = h->partials;
+        b.partials = part;
         b.src = h->pop[h->cur];
         b.side_in = h->side[h->sc];
         rc = launch_step(h, 1, b);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, h->stream, h->partials, nblk, h->mass);
+    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, h->stream, part, nblk, h->mass);
     FD_HIP(hipGetLastError());
     return FD_OK;
 }
@@ -1773,6 +1790,55 @@ int fd_phase_mass(fd_handle* h, double out[3]) {
     if (!h || !out) return fail(FD_E_ARG, "handle/out is NULL");
     FD_HIP(hipSetDevice(h->device));
     int rc = compute_fields(h, nullptr);
+    if (rc) return rc;
+    FD_HIP(hipMemcpyAsync(out, h->mass, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
+    FD_HIP(hipStreamSynchronize(h->stream));
+    return FD_OK;
+}
+
+}  // extern "C"
+
+// Fixed FP64 partial-sum scratch of fd_phase_mass_noalloc: kScratchSlots
+// slots of 4 doubles (32 B), 4 MiB.  A device global: every GPU the library
+// runs on holds its own copy, at its own address, until the process ends.
+// In host code `scratch` names a host-side stand-in, never the device
+// memory: kernels and copies get the address from scratch_of().
+namespace {
+constexpr int kScratchSlots = 131072;
+}
+static __device__ __align__(256) double scratch[4 * kScratchSlots];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of `scratch` on device dev (the current device),
+// looked up once per device.
+int scratch_of(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(FD_E_ARG, "device index out of range for the fixed scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fd_phase_mass_noalloc(fd_handle* h, double out[3]) {
+    if (!h || !out) return fail(FD_E_ARG, "handle/out is NULL");
+    FD_HIP(hipSetDevice(h->device));
+    double* sp = nullptr;
+    int rc = scratch_of(h->device, &sp);
+    if (rc) return rc;
+    rc = compute_fields(h, nullptr, sp, kScratchSlots);
     if (rc) return rc;
     FD_HIP(hipMemcpyAsync(out, h->mass, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
     FD_HIP(hipStreamSynchronize(h->stream));

@@ -155,6 +155,17 @@ int fd_set_populations(fd_handle* h, const float* f, const float* g);
  * Synchronises. */
 int fd_phase_mass(fd_handle* h, double out[3]);
 
+/* fd_phase_mass without any allocation: the FP64 partial sums go to a fixed
+ * 4 MiB scratch array inside the library (131,072 slots of 32 B: one per
+ * wave of the moments launch, or per 256 nodes before the first iteration),
+ * one copy per GPU, held until the process ends.  No reference counterpart:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  A launch
+ * shape (fd_set_tile) that needs more slots fails with FD_E_ARG.  Every call
+ * on one GPU uses that GPU's array: calls that overlap on one GPU (other
+ * host threads or handles) race.  Synchronises. */
+int fd_phase_mass_noalloc(fd_handle* h, double out[3]);
+
 /* Failure watchdog of the last completed iteration, FP64:
  * out[0] = number of fluid nodes whose rho or psi is not finite,
  * out[1] = min over fluid nodes of tau_mix - 1/2 (FP32 per node).  BGK needs
