@@ -566,21 +566,23 @@ __global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
 // S_n+1 kept in registers and never written to HBM, so each iteration moves
 // half the bytes of a single-iteration launch.
 //
-// A wave owns a strip of kFuseCols output columns [xb0, xb0 + 60) and its
-// 64 lanes hold columns a0 = xb0 - 2 ... xb0 + 61.  Stage A is the
-// single-iteration march on all 64 lanes (S_n from HBM -> S_n+1 of the
-// strip, P_n+1 of the Zou-He columns); stage B marches two rows behind it
-// and pulls S_n+1 from A's last rows in registers: a population moving in
-// x comes from the neighbouring lane by a DPP lane shift (wave_shr:1 /
-// wave_shl:1).  B needs A's output two columns either side of a node (one
+// A wave owns a strip of kFuseCols output columns [xb0, xb0 + 58) and its
+// 64 lanes hold columns a0 = xb0 - 3 ... xb0 + 60.  Stage A is the
+// single-iteration march (S_n from HBM -> S_n+1 of the strip, P_n+1 of the
+// Zou-He columns), valid on lanes 1..62 (its psi stencil takes the
+// neighbours' psi from the neighbouring lanes); stage B marches two rows behind it
+// and pulls S_n+1 from A's last rows: g from registers, where a population
+// moving in x comes from the neighbouring lane by a DPP lane shift
+// (wave_shr:1 / wave_shl:1), f from a four-row ring in LDS read at the
+// shifted lane.  B needs A's output two columns either side of a node (one
 // for the f pull, one more for the g pulls of the psi stencil), so it is
-// valid on lanes 2..61: strips overlap by four columns of stage-A work.
+// valid on lanes 3..60: strips overlap by six columns of stage-A work.
 // For the chunk of output rows [y0, y1) stage A runs rows y0-2 ... y1+1.
 //
 // Every node of either stage runs the single-iteration kernel's arithmetic
 // on the same values in the same order, so the result equals two
 // single-iteration launches bit for bit.
-constexpr int kFuseCols = 60;
+constexpr int kFuseCols = 58;
 
 __device__ __forceinline__ float lane_from_left(float v) {  // lane l <- lane l-1 (DPP wave_shr:1)
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
@@ -588,14 +590,24 @@ __device__ __forceinline__ float lane_from_left(float v) {  // lane l <- lane l-
 __device__ __forceinline__ float lane_from_right(float v) {  // lane l <- lane l+1 (DPP wave_shl:1)
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
 }
+// psi of a row at this lane and its left / right neighbour lanes (lanes 0
+// and 63 get 0 for the missing neighbour; their stencils are not used)
+__device__ __forceinline__ PsiRow lane_row(float c) {
+    PsiRow p;
+    p.c = c;
+    p.l = lane_from_left(c);
+    p.r = lane_from_right(c);
+    return p;
+}
+
 // value v of the lane at column x - ex (ex in {-1, 0, 1})
 __device__ __forceinline__ float lane_pull(float v, int ex) {
     return ex > 0 ? lane_from_left(v) : (ex < 0 ? lane_from_right(v) : v);
 }
 
 // Stage B's pull of one distribution for the node of row y at this lane
-// from stage A's post-collision rows y-1 (lo, only directions 2, 5, 6 are
-// read), y (mid) and y+1 (hi); a linked direction i takes the node's own
+// (g) from stage A's post-collision rows y-1 (lo, only directions 2, 5, 6
+// are read), y (mid) and y+1 (hi); a linked direction i takes the node's own
 // mid[opp(i)] (half-way bounce-back).  Lanes with !on get 0 (inert solids,
 // ghost columns), as the single-iteration kernel's unloaded lanes do.
 __device__ __forceinline__ void reg_pull(const float lo[9], const float mid[9], const float hi[9], unsigned links,
@@ -609,13 +621,34 @@ __device__ __forceinline__ void reg_pull(const float lo[9], const float mid[9], 
     }
 }
 
+// Stage A's post-collision f of the last four rows, for stage B, in LDS:
+// per wave 4 slots (row & 3) x 9 planes x 64 lanes, with one float of
+// padding at either end of the wave's region for the lane-shifted reads of
+// lanes 0 and 63 (their values are never used).
+constexpr int kRingSlot = 9 * 64;
+constexpr int kRingRegion = 4 * kRingSlot + 2;
+
+// Stage B's pull of f for the node of row y at this lane from the ring:
+// direction i from row y - e_iy at lane - e_ix, or, linked, the node's own
+// opposite population of row y (half-way bounce-back) - one LDS read at one
+// of two addresses.  Lanes with !on get 0.
+__device__ __forceinline__ void lds_pull(const float* ring, int y, int lane, unsigned links, bool on, float v[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        int idx = ((y - fdn::EY[i]) & 3) * kRingSlot + i * 64 + lane - fdn::EX[i];
+        if (i > 0) idx = ((links >> (i - 1)) & 1u) ? (y & 3) * kRingSlot + fdn::OPP[i] * 64 + lane : idx;
+        const float x = ring[idx];
+        v[i] = on ? x : 0.f;
+    }
+}
+
 // Inlet density (fingering_periodic.py:276-277) of the node of row y at
-// column 0 from stage A's rows y-1, y, y+1: the adjacent-row density of the
-// walled inlet corners (fingering.py:334-364).  Every lane executes it.
-__device__ __forceinline__ float reg_inlet_rho(const float lo[9], const float mid[9], const float hi[9],
-                                               unsigned links, float u, float px) {
+// column 0 from stage A's rows y-1, y, y+1 in the ring: the adjacent-row
+// density of the walled inlet corners (fingering.py:334-364).
+__device__ __forceinline__ float ring_inlet_rho(const float* ring, int y, int lane, unsigned links, float u,
+                                                float px) {
     float f[9];
-    reg_pull(lo, mid, hi, links, true, f);
+    lds_pull(ring, y, lane, links, true, f);
     return fdn::inlet_rho(f, u, px);
 }
 
@@ -669,12 +702,10 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     const int strip = (L - chunk * k.nsb) * WPB + wave;
     const int xb0 = strip * kFuseCols;
     if (xb0 >= k.W) return;
-    const int a0 = xb0 - 2;
+    const int a0 = xb0 - 3;
     const int xa = a0 + lane;
     const Lane ln = lane_of(k, xa);
-    const Lane lh = lane_of(k, lane == 0 ? a0 - 1 : a0 + 64);
-    const bool hl = (lane == 0) || (lane == 63);
-    const bool bout = lane >= 2 && lane < 2 + kFuseCols && xa < k.W;   // stage B stores this lane
+    const bool bout = lane >= 3 && lane < 3 + kFuseCols && xa < k.W;   // stage B stores this lane
     const bool zx = !k.x_periodic;
     const bool has_in = zx && xb0 == 0;                                  // column 0 among the outputs
     const int y0 = chunk * k.rows;
@@ -684,6 +715,8 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     const rsrc_t src = make_rsrc(b.src, bytes);
     const rsrc_t dst = make_rsrc(b.dst, bytes);
     const PullOff po = pull_offsets(k, ln.x);
+    __shared__ float fring_all[WPB * kRingRegion];
+    float* const fring = fring_all + wave * kRingRegion + 1;
 
     // ---- stage A prologue: psi_n+1 of rows y0-3, y0-2 and g of row y0-2
     const int ys = y0 - 2;
@@ -691,31 +724,26 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     PsiRow pb, pc;
     {
         const int yb = ys - 1;
-        const unsigned fb = flags_of(k, b, ln, yb), fbh = flags_of(k, b, lh, yb);
+        const unsigned fb = flags_of(k, b, ln, yb);
         const bool on = g_on(k, ln, yb, fb);
         pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
-        float h = 0.f;
-        if (hl) h = psi_halo(k, src, lh, yb, fbh);
-        pb = psi_row_of(finish_g(k, ln, yb, fb, on, gc), h, lane);
+        pb = lane_row(finish_g(k, ln, yb, fb, on, gc));
     }
     unsigned flc = flags_of(k, b, ln, ys);
     {
-        const unsigned fch = flags_of(k, b, lh, ys);
         const bool on = g_on(k, ln, ys, flc);
         pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, gc);
-        float h = 0.f;
-        if (hl) h = psi_halo(k, src, lh, ys, fch);
-        pc = psi_row_of(finish_g(k, ln, ys, flc, on, gc), h, lane);
+        pc = lane_row(finish_g(k, ln, ys, flc, on, gc));
     }
     unsigned fla = flags_of(k, b, ln, ys + 1);
-    unsigned flah = flags_of(k, b, lh, ys + 1);
 
-    // ---- the hand-over ring: A's post-collision rows r (Fa, Ga), r-1
-    // (Fb, Gb), r-2 (Fc; Gc only directions 2, 5, 6), r-3 (Fd: 2, 5, 6);
-    // A's flags of rows r, r-1, r-2; A's side values P of rows r, r-1, r-2
-    float Fa[9], Fb[9], Fc[9], Fd[9], Ga[9], Gb[9], Gc[9];
+    // ---- the hand-over: A's post-collision f of rows r .. r-3 in the LDS
+    // ring; its g of rows r (Ga), r-1 (Gb), r-2 (Gc: only directions 2, 5, 6
+    // are read) in registers; A's flags of rows r, r-1, r-2; A's side values
+    // P of rows r, r-1, r-2
+    float Ga[9], Gb[9], Gc[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fa[i] = Fb[i] = Fc[i] = Fd[i] = Ga[i] = Gb[i] = Gc[i] = 0.f;
+    for (int i = 0; i < 9; ++i) Ga[i] = Gb[i] = Gc[i] = 0.f;
     unsigned fr0 = kSolid, fr1 = kSolid, fr2 = kSolid;
     float pxa = 0.f, pya = 0.f, pxb = 0.f, pyb = 0.f, pxc = 0.f, pyc = 0.f;
     // ---- stage B state: psi_n+2 of rows y-1 (qs), y (qc); g of row y (gq);
@@ -732,19 +760,16 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
         const bool arow = !(k.y_walls && (r < 0 || r >= k.H));
         const int rw = wrap(r, k.H);
         // ---------------- stage A, row r
-        float Fn[9], Gn[9], pxn = 0.f, pyn = 0.f;
+        float Gn[9], pxn = 0.f, pyn = 0.f;
         {
             const int ya = r + 1;
             const unsigned fl2 = flags_of(k, b, ln, ya + 1);
-            const unsigned fl2h = flags_of(k, b, lh, ya + 1);
             float ga[9], f[9];
             const bool ga_on = g_on(k, ln, ya, fla);
             pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
             const bool f_on = arow && ln.inside && !((flc & kSolid) && k.inert);
             pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, f_on, f);
-            float h = 0.f;
-            if (hl) h = psi_halo(k, src, lh, ya, flah);
-            const PsiRow pa = psi_row_of(finish_g(k, ln, ya, fla, ga_on, ga), h, lane);
+            const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, ga_on, ga));
             const float nb9[9] = {pc.c, pc.r, pa.c, pc.l, pb.c, pa.r, pa.l, pb.l, pb.r};
             float dpx, dpy, lap;
             fdn::stencil(nb9, dpx, dpy, lap);
@@ -753,8 +778,9 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
             for (int i = 0; i < 9; ++i) Gn[i] = gc[i];
             const bool solid = !arow || !ln.inside || (flc & kSolid) != 0;
             const float mu = collide_node(k, f, Gn, pc.c, dpx, dpy, lap, solid);
+            float* const slot = fring + (r & 3) * kRingSlot + lane;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Fn[i] = f[i];
+            for (int i = 0; i < 9; ++i) slot[i * 64] = f[i];
             // P = d psi * mu of this row's psi at the Zou-He columns (the
             // value the next iteration's Zou-He reads)
             pxn = dpx * mu;
@@ -768,15 +794,10 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
             fr0 = arow && ln.inside ? flc : kSolid;
             flc = fla;
             fla = fl2;
-            flah = fl2h;
         }
         // ---------------- hand-over
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            Fd[i] = Fc[i];
-            Fc[i] = Fb[i];
-            Fb[i] = Fa[i];
-            Fa[i] = Fn[i];
             Gc[i] = Gb[i];
             Gb[i] = Ga[i];
             Ga[i] = Gn[i];
@@ -795,25 +816,22 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
             const int yq = r - 1;
             const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr1 & kSolid) && k.inert);
             reg_pull(Gc, Gb, Ga, fr1 & kLinks, on, gn);
-            const float q = finish_g(k, ln, yq, fr1, on, gn);
-            qn.c = q;
-            qn.l = lane_from_left(q);
-            qn.r = lane_from_right(q);
+            qn = lane_row(finish_g(k, ln, yq, fr1, on, gn));
         }
         // walled inlet corners need the inlet density of the adjacent row:
         // row y0-1 is the one stage B has not marched (top corner at y0 = H-1)
         if (j == 3 && has_in && k.y_walls && y0 >= 1)
-            rho_prev = reg_inlet_rho(Fd, Fc, Fb, fr2 & kLinks, b.uprof[y0 - 1], pxc);
+            rho_prev = ring_inlet_rho(fring, y0 - 1, lane, fr2 & kLinks, b.uprof[y0 - 1], pxc);
         if (j >= 4) {
             // ------------ stage B: collision of row y = r-2
             const int y = r - 2;
             const unsigned fl = fr2;
             const bool solid = (fl & kSolid) != 0;
             float f[9];
-            reg_pull(Fd, Fc, Fb, fl & kLinks, ln.inside && !(solid && k.inert), f);
+            lds_pull(fring, y, lane, fl & kLinks, ln.inside && !(solid && k.inert), f);
             float rho_nb = rho_prev;
             if (has_in) {
-                if (k.y_walls && y == 0) rho_nb = reg_inlet_rho(Fc, Fb, Fa, fr1 & kLinks, b.uprof[1], pxb);
+                if (k.y_walls && y == 0) rho_nb = ring_inlet_rho(fring, 1, lane, fr1 & kLinks, b.uprof[1], pxb);
                 rho_prev = fdn::inlet_rho(f, b.uprof[y], pxc);
             }
             const float nb9[9] = {qc.c, qc.r, qn.c, qc.l, qs.c, qn.r, qn.l, qs.l, qs.r};
@@ -1137,7 +1155,7 @@ struct fd_handle {
     int rows = 16;  // rows per wave
     int fuse = 2;   // iterations per launch of fd_step (1, or 2: fused_kernel)
     int fwx = 1;    // fused_kernel: waves per workgroup
-    int frows = 24; // fused_kernel: output rows per wave
+    int frows = 16; // fused_kernel: output rows per wave
     int timed_launches = 0;
     bool have_timing = false;
     int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
