@@ -608,16 +608,18 @@ __device__ __forceinline__ float lane_pull(float v, int ex) {
 // Stage B's pull of one distribution for the node of row y at this lane
 // (g) from stage A's post-collision rows y-1 (lo, only directions 2, 5, 6
 // are read), y (mid) and y+1 (hi); a linked direction i takes the node's own
-// mid[opp(i)] (half-way bounce-back).  Lanes with !on get 0 (inert solids,
-// ghost columns), as the single-iteration kernel's unloaded lanes do.
+// mid[opp(i)] (half-way bounce-back).  An inert solid node has every link
+// bit set and stage A left zeros in it, so it pulls zeros, as the
+// single-iteration kernel's unloaded lanes do; lanes on ghost columns pull
+// values that are never used.
 __device__ __forceinline__ void reg_pull(const float lo[9], const float mid[9], const float hi[9], unsigned links,
-                                         bool on, float v[9]) {
+                                         float v[9]) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const float* row = fdn::EY[i] > 0 ? lo : (fdn::EY[i] < 0 ? hi : mid);
         float x = lane_pull(row[i], fdn::EX[i]);
         if (i > 0) x = ((links >> (i - 1)) & 1u) ? mid[fdn::OPP[i]] : x;
-        v[i] = on ? x : 0.f;
+        v[i] = x;
     }
 }
 
@@ -631,14 +633,13 @@ constexpr int kRingRegion = 4 * kRingSlot + 2;
 // Stage B's pull of f for the node of row y at this lane from the ring:
 // direction i from row y - e_iy at lane - e_ix, or, linked, the node's own
 // opposite population of row y (half-way bounce-back) - one LDS read at one
-// of two addresses.  Lanes with !on get 0.
-__device__ __forceinline__ void lds_pull(const float* ring, int y, int lane, unsigned links, bool on, float v[9]) {
+// of two addresses.  Inert solid nodes pull zeros (see reg_pull).
+__device__ __forceinline__ void lds_pull(const float* ring, int y, int lane, unsigned links, float v[9]) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         int idx = ((y - fdn::EY[i]) & 3) * kRingSlot + i * 64 + lane - fdn::EX[i];
         if (i > 0) idx = ((links >> (i - 1)) & 1u) ? (y & 3) * kRingSlot + fdn::OPP[i] * 64 + lane : idx;
-        const float x = ring[idx];
-        v[i] = on ? x : 0.f;
+        v[i] = ring[idx];
     }
 }
 
@@ -648,7 +649,7 @@ __device__ __forceinline__ void lds_pull(const float* ring, int y, int lane, uns
 __device__ __forceinline__ float ring_inlet_rho(const float* ring, int y, int lane, unsigned links, float u,
                                                 float px) {
     float f[9];
-    lds_pull(ring, y, lane, links, true, f);
+    lds_pull(ring, y, lane, links, f);
     return fdn::inlet_rho(f, u, px);
 }
 
@@ -815,7 +816,7 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
         {
             const int yq = r - 1;
             const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr1 & kSolid) && k.inert);
-            reg_pull(Gc, Gb, Ga, fr1 & kLinks, on, gn);
+            reg_pull(Gc, Gb, Ga, fr1 & kLinks, gn);
             qn = lane_row(finish_g(k, ln, yq, fr1, on, gn));
         }
         // walled inlet corners need the inlet density of the adjacent row:
@@ -828,7 +829,7 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
             const unsigned fl = fr2;
             const bool solid = (fl & kSolid) != 0;
             float f[9];
-            lds_pull(fring, y, lane, fl & kLinks, ln.inside && !(solid && k.inert), f);
+            lds_pull(fring, y, lane, fl & kLinks, f);
             float rho_nb = rho_prev;
             if (has_in) {
                 if (k.y_walls && y == 0) rho_nb = ring_inlet_rho(fring, 1, lane, fr1 & kLinks, b.uprof[1], pxb);
@@ -1155,7 +1156,7 @@ struct fd_handle {
     int rows = 16;  // rows per wave
     int fuse = 2;   // iterations per launch of fd_step (1, or 2: fused_kernel)
     int fwx = 1;    // fused_kernel: waves per workgroup
-    int frows = 16; // fused_kernel: output rows per wave
+    int frows = 12; // fused_kernel: output rows per wave
     int timed_launches = 0;
     bool have_timing = false;
     int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
@@ -1245,16 +1246,18 @@ bool pack_flags(const fd_params& p, const uint8_t* solid, const uint8_t* links, 
             unsigned v = links[(size_t)y * W + x] & FD_LINK_BITS;
             if (walls && y == p.wall_lo) v |= kWallLoLinks;
             if (walls && y == p.wall_hi) v |= kWallHiLinks;
-            if (sol(y, x)) v = inert ? kSolid : (v | kSolid);
+            // an inert solid node is never loaded; all its link bits set make
+            // a register pull of its populations read its own (zero) ones
+            if (sol(y, x)) v = inert ? (kSolid | kLinks) : (v | kSolid);
             fl[(size_t)y * Wp + x] = (uint16_t)v;
         }
     if (inert)
         for (int y = 0; y < H; ++y)
             for (int x0 = 0; x0 < Wp; x0 += 16) {
                 bool dead = true;
-                for (int x = x0; x < x0 + 16 && dead; ++x) dead = fl[(size_t)y * Wp + x] == kSolid;
+                for (int x = x0; x < x0 + 16 && dead; ++x) dead = (fl[(size_t)y * Wp + x] & kSolid) != 0;
                 if (dead)
-                    for (int x = x0; x < x0 + 16; ++x) fl[(size_t)y * Wp + x] = (uint16_t)(kSolid | kDeadPiece);
+                    for (int x = x0; x < x0 + 16; ++x) fl[(size_t)y * Wp + x] |= (uint16_t)kDeadPiece;
             }
     return inert;
 }
