@@ -179,6 +179,60 @@ def ensemble_bench(n, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+def cfg1_bench(steps):
+    """The 400 x 100 channel of the reference-faithful configuration
+    (periodic y, Gaussian Zou-He inlet/outlet, 56 seeded 6x6 squares, M = 5,
+    psi_wall = 0): `steps` iterations, one launch per iteration or two, eager
+    or replayed from a captured graph, each at the library's default launch
+    shape for this size (one row per wave).  Its state (6 MB) stays in the
+    caches, so this measures launch overhead and latency, not HBM.  Prints
+    one JSON line."""
+    from fingering_dynamics_amd.lbm import Compute
+    from fingering_dynamics_amd.params import LbmParams
+    from fingering_dynamics_amd import geometry
+    h, w = 100, 400
+    rects = geometry.random_squares(h, w, 56, size=6, seed=1, x_min=41)
+    bpa, corners = geometry.Createblock(h, w).setblock(rects)
+    mask = geometry.fluid_mask(bpa)
+    bits = geometry.Bounce_back(h, w).link_bits_rec(corners)
+    params = LbmParams.periodic(h, w, M=5.0, psi_wall=0.0, n_inv_cols=40)
+    res = {}
+    for depth in (2, 1):
+        for graph in (True, False):
+            cm = Compute(mask, bits, params)
+            if depth == 1:
+                cm.set_fusion(1)
+            cm.use_graph(graph)
+            cm.step(100)
+            cm.synchronize()
+            t0 = time.perf_counter()
+            cm.step(steps)
+            cm.synchronize()
+            dt = time.perf_counter() - t0
+            res["%s_%s" % ("two_per_launch" if depth == 2 else "one_per_launch", "graph" if graph else "eager")] = {
+                "us_per_iteration": round(dt * 1e6 / steps, 3), "device_us_per_iteration": round(cm.last_step_ms() * 1e3, 3),
+                "mlups": round(h * w * steps / dt / 1e6, 1)}
+            bad = cm.mass()[2]
+            cm.close()
+    from oracle import lbm64
+    o = lbm64.Lbm64.from_params(params, mask, geometry.unpack_links(bits))
+    o.step(1)
+    n_cpu = 200
+    t0 = time.perf_counter()
+    o.step(n_cpu)
+    cpu_dt = time.perf_counter() - t0
+    best = max(res, key=lambda k: res[k]["mlups"])
+    print(json.dumps({
+        "metric": "MLUPS of the 400x100 channel (%d iterations), by launch mode" % steps,
+        "value": res[best]["mlups"], "unit": "MLUPS", "best_mode": best, "modes": res,
+        "graph_speedup_two_per_launch": round(res["two_per_launch_graph"]["mlups"] / res["two_per_launch_eager"]["mlups"], 2),
+        "non_finite_after": bad,
+        "cpu_baseline": {"value": round(h * w * n_cpu / cpu_dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+                         "sample": "%d iterations of the same channel, FP64 numpy restatement, %.1f s" % (n_cpu, cpu_dt)},
+        "config": {"workload": "400x100 channel, periodic y, Gaussian Zou-He, 56 squares 6x6, M=5, psi_wall=0"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +245,8 @@ def main():
     ap.add_argument("--ftile", default="", help="launch shape WX,ROWS of the two-iteration kernel")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
+    ap.add_argument("--cfg1", action="store_true",
+                    help="instead: the 400x100 channel, eager vs graph-replayed launches (--steps, default 5000)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="instead: the many-domain workload with N domains, batched vs sequential launches")
     args = ap.parse_args()
@@ -198,6 +254,9 @@ def main():
         ms, gbps = copy_probe()
         print(json.dumps({"probe": "streaming copy (fd_copy_probe)", "bytes_read": 1 << 30,
                           "bytes_written": 1 << 30, "ms_per_copy": ms, "GBps": gbps}), flush=True)
+        return
+    if args.cfg1:
+        cfg1_bench(args.steps if args.steps != 20000 else 5000)
         return
     if args.ensemble:
         ensemble_bench(args.ensemble, args.steps if args.steps != 20000 else 2000, args.warmup if args.warmup != 500 else 20)

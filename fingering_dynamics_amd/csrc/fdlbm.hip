@@ -1155,6 +1155,9 @@ struct fd_handle {
     int wx = 4;     // waves (64-column strips) per workgroup
     int rows = 16;  // rows per wave
     int fuse = 2;   // iterations per launch of fd_step (1, or 2: fused_kernel)
+    bool use_graph = false;            // fd_step replays captured launch pairs
+    hipGraphExec_t gexec = nullptr;    // the captured pair
+    int gkey[5] = {0, 0, 0, 0, 0};     // its launch configuration
     int fwx = 1;    // fused_kernel: waves per workgroup
     int frows = 12; // fused_kernel: output rows per wave
     int timed_launches = 0;
@@ -1364,6 +1367,48 @@ int launch_fused(fd_handle* h, const KBufs& b) {
     return FD_OK;
 }
 
+// Buffers of one launch from population parity c and side parity sc.
+KBufs step_bufs(const fd_handle* h, int c, int sc) {
+    KBufs b{};
+    b.src = h->pop[c];
+    b.dst = h->pop[1 - c];
+    b.flags = h->flags;
+    b.side_in = h->side[sc];
+    b.side_out = h->side[1 - sc];
+    b.uprof = h->uprof;
+    return b;
+}
+
+// The captured pair of launches of `per` iterations each, from parity 0:
+// (re)captured when the launch configuration changed since the last one.
+int step_graph(fd_handle* h, int per) {
+    const int key[5] = {per, h->fwx, h->frows, h->wx, h->rows};
+    if (h->gexec && std::memcmp(key, h->gkey, sizeof key) == 0) return FD_OK;
+    if (h->gexec) {
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    FD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FD_OK;
+    for (int p = 0; p < 2 && !rc; ++p)
+        rc = per == 2 ? launch_fused(h, step_bufs(h, p, p)) : launch_step(h, 0, step_bufs(h, p, p));
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(FD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        h->gexec = nullptr;
+        return fail(FD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    std::memcpy(h->gkey, key, sizeof key);
+    return FD_OK;
+}
+
 // Number of FP64 partial-sum slots (one per wave) a moments launch writes.
 int moments_grid_blocks(const fd_handle* h) { return h->wx * step_grid(h, nullptr); }
 
@@ -1493,6 +1538,7 @@ void fd_destroy(fd_handle* h) {
     if (h->fields) (void)hipFree(h->fields);
     if (h->partials) (void)hipFree(h->partials);
     if (h->mass) (void)hipFree(h->mass);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->evs) (void)hipEventDestroy(h->evs);
@@ -1522,9 +1568,17 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     std::vector<uint16_t> fl;
     h->inert = pack_flags(*p, solid, links, h->Wp, fl);
     fill_kparams(h);
-    // A large domain favours 4-wave strips; small ones need more workgroups.
-    h->wx = 4;
-    h->rows = 16;
+    // Launch shapes: rows per wave chosen so that a launch has enough waves
+    // to fill the device (about 8192 for one iteration per launch, 12288 for
+    // two; measured on MI355X: 4 x 16 and 1 x 12 on 4096 x 2048, 1 row per
+    // wave on 400 x 100).  fd_set_tile / fd_set_fusion override them.
+    {
+        const int64_t strips64 = (p->W + 63) / 64, strips58 = (p->W + kFuseCols - 1) / kFuseCols;
+        h->wx = strips64 >= 16 ? 4 : 1;
+        h->rows = (int)std::min<int64_t>(16, std::max<int64_t>(1, (strips64 * p->H + 4096) / 8192));
+        h->fwx = 1;
+        h->frows = (int)std::min<int64_t>(12, std::max<int64_t>(1, (strips58 * p->H + 6144) / 12288));
+    }
 
     const size_t Q = h->Q, H = p->H;
     std::vector<float> up(H);
@@ -1640,32 +1694,40 @@ int fd_step(fd_handle* h, int32_t nsteps) {
     // launched_through(h); reaching iteration N' needs launches up to N'-1.
     const int64_t have = launched_through(h);
     const int64_t nl = launches_for(h, nsteps);
-    if (nl > 0) FD_HIP(hipEventRecord(h->ev0, h->stream));
-    for (int64_t i = 0; i < nl;) {
-        KBufs b{};
-        b.src = h->pop[h->cur];
-        b.dst = h->pop[1 - h->cur];
-        b.flags = h->flags;
-        b.side_in = h->side[h->sc];
-        b.side_out = h->side[1 - h->sc];
-        b.uprof = h->uprof;
-        // pairs of iterations go to the two-iteration kernel
-        const int it = (fusable(h) && nl - i >= 2) ? 2 : 1;
-        int rc = it == 2 ? launch_fused(h, b) : launch_step(h, 0, b);
-        if (rc) {
-            // the i launches already issued stand: the buffers hold the
-            // post-collision state of iteration have + i
-            if (i > 0) {
-                h->N = have + i + 1;
-                h->stage = 0;
-            }
-            h->have_timing = false;
-            return rc;
+    int64_t i = 0;  // iterations issued
+    auto failed = [&](int rc) {
+        // the launches already issued stand: the buffers hold the
+        // post-collision state of iteration have + i
+        if (i > 0) {
+            h->N = have + i + 1;
+            h->stage = 0;
         }
+        h->have_timing = false;
+        return rc;
+    };
+    auto issue = [&](int it) {
+        int rc = it == 2 ? launch_fused(h, step_bufs(h, h->cur, h->sc)) : launch_step(h, 0, step_bufs(h, h->cur, h->sc));
+        if (rc) return rc;
         h->cur ^= 1;
         h->sc ^= 1;
         i += it;
+        return FD_OK;
+    };
+    if (nl > 0) FD_HIP(hipEventRecord(h->ev0, h->stream));
+    // pairs of iterations go to the two-iteration kernel; long loops replay
+    // a captured pair of launches (buffer parity 0 -> 1 -> 0)
+    const int per = fusable(h) ? 2 : 1;
+    int rc = FD_OK;
+    if (h->use_graph && h->cur == h->sc && nl >= 4 * per) {
+        if (h->cur != 0 && (rc = issue(per))) return failed(rc);
+        if ((rc = step_graph(h, per))) return failed(rc);
+        for (; nl - i >= 2 * per; i += 2 * per) {
+            const hipError_t e = hipGraphLaunch(h->gexec, h->stream);
+            if (e != hipSuccess) return failed(fail(FD_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e)));
+        }
     }
+    while (i < nl)
+        if ((rc = issue(per == 2 && nl - i >= 2 ? 2 : 1))) return failed(rc);
     if (nl > 0) {
         FD_HIP(hipEventRecord(h->ev1, h->stream));
         h->timed_launches = (int)nl;
@@ -1673,6 +1735,12 @@ int fd_step(fd_handle* h, int32_t nsteps) {
     }
     h->N += nsteps;
     if (nsteps > 0) h->stage = 0;
+    return FD_OK;
+}
+
+int fd_use_graph(fd_handle* h, int32_t on) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    h->use_graph = on != 0;
     return FD_OK;
 }
 

@@ -122,12 +122,43 @@ class Compute:
         marching `rows` rows of its own 64-column strip."""
         _native.check(self._lib.fd_set_tile(self._h, int(waves_x), int(rows)))
 
+    def use_graph(self, on=True):
+        """Replay long step loops from a captured HIP graph, or launch every
+        iteration eagerly (the default); identical results."""
+        _native.check(self._lib.fd_use_graph(self._h, 1 if on else 0))
+
     def set_fusion(self, depth, waves_x=0, rows=0):
         """Iterations per launch: 2 (default) keeps the intermediate state of
         each pair of iterations in registers; 1 runs one launch per
         iteration.  Identical results.  waves_x / rows (> 0) set the
         two-iteration kernel's launch shape."""
         _native.check(self._lib.fd_set_fusion(self._h, int(depth), int(waves_x), int(rows)))
+
+    TUNE_CANDIDATES = tuple((2, 1, r) for r in (1, 2, 4, 8, 12, 16, 24)) + ((2, 4, 16),) + \
+        tuple((1, wx, r) for wx, r in ((1, 1), (4, 1), (4, 4), (4, 16)))
+
+    def autotune(self, candidates=TUNE_CANDIDATES, steps=20):
+        """Time ``steps`` real iterations per launch configuration
+        (depth, waves, rows) and keep the fastest; the results do not depend
+        on it, so the trial iterations are part of the run.  Returns
+        {(depth, waves, rows): ms per iteration}."""
+        times = {}
+        for depth, wx, rows in candidates:
+            if depth == 1:
+                self.set_fusion(1)
+                self.set_tile(wx, rows)
+            else:
+                self.set_fusion(2, wx, rows)
+            self.step(2)
+            self.step(steps)
+            times[(depth, wx, rows)] = self.last_step_ms()
+        best = min(times, key=times.get)
+        if best[0] == 1:
+            self.set_fusion(1)
+            self.set_tile(*best[1:])
+        else:
+            self.set_fusion(2, *best[1:])
+        return times
 
     # ------------------------------------------------------------ outputs
     def fields(self):

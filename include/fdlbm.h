@@ -113,6 +113,13 @@ int fd_init_fields(fd_handle* h, const double* psi, const double* rho, const dou
  * waiting. */
 int fd_step(fd_handle* h, int32_t nsteps);
 
+/* 1: fd_step replays long step loops from a captured HIP graph of two
+ * launches; 0 (default): every launch is issued eagerly.  Identical results.
+ * Measured on MI355X, replay is 1-2 us per iteration slower than eager
+ * launches on a 400 x 100 channel and the same on large domains.  No
+ * reference counterpart (a launch control of this implementation). */
+int fd_use_graph(fd_handle* h, int32_t on);
+
 /* Number of iterations done since fd_init. */
 int64_t fd_steps_done(const fd_handle* h);
 
@@ -219,7 +226,8 @@ double fd_last_step_ms(const fd_handle* h);
 
 /* Kernel launch shape: workgroups of waves_x waves (1, 2, 4 or 8), each
  * wave marching `rows` rows (>= 1) of its own 64-column strip.  The default
- * is 4 x 16.  No reference counterpart (a launch-shape control of this
+ * is 4 x 16, fewer rows (and 1 wave) on domains too small to fill the
+ * device.  No reference counterpart (a launch-shape control of this
  * implementation). */
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
 
@@ -228,7 +236,8 @@ int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
  * registers, so each iteration reads and writes HBM once per two
  * iterations; depth 1 runs one launch per iteration.  The results are
  * identical bit for bit.  waves_x (1, 2 or 4) and rows set the two-iteration
- * kernel's launch shape (<= 0 keeps the current one; default 1 x 12).
+ * kernel's launch shape (<= 0 keeps the current one; default 1 wave x 12 rows,
+ * fewer rows on domains too small to fill the device).
  * Domains with H < 8, and walled domains whose walls are not the edge rows
  * (the rows 1 and H-2 of fingering.py:573), always run depth 1.  No reference counterpart (a
  * launch control of this implementation). */

@@ -127,3 +127,23 @@ def test_fused_equals_single_after_state_hand_over():
     two.set_populations(f, g)
     _same(one, two, "handed over")
     _run(one, two, (2, 3), "after hand-over")
+
+
+def test_graph_replay_equals_eager_launches():
+    """Replayed launch pairs (fd_use_graph, the default) against eager
+    launches, over several fd_step calls with reads, a launch-shape change
+    and a depth change in between; every replay is checked."""
+    params, mask, bits, _ = _setup("walled_pressure")
+    eager = _compute(mask, bits, params)
+    eager.use_graph(False)
+    graph = _compute(mask, bits, params)
+    for n, change in ((9, None), (16, None), (5, ("fusion", (1, 0, 0))), (12, None), (7, ("fusion", (2, 1, 5))),
+                      (20, ("tile", (2, 3))), (8, None), (1, None), (13, None)):
+        if change:
+            for cm in (eager, graph):
+                getattr(cm, "set_" + change[0])(*change[1])
+        eager.step(n)
+        graph.step(n)
+        _same(eager, graph, "graph t=%d" % eager.t)
+    eager.close()
+    graph.close()

@@ -70,3 +70,50 @@ def test_cfg1_trajectory_5000():
               % ((g_rho - a_rho) / a_rho, (b_rho - a_rho) / a_rho, (g_psi - a_psi) / nf, (b_psi - a_psi) / nf),
               flush=True)
     cm.close()
+
+
+def test_cfg1b_trajectory_5000():
+    """The configuration-literal channel: periodic in x with a constant body
+    force g_x = 1e-5 and half-way walls on rows 0 and H-1 (no inlet or
+    outlet: a closed box, so sum rho and sum psi are invariants), the same
+    56 squares, M = 5, psi_wall = 0, psi = +1 in the first 40 columns.  The
+    reference has no body force and no periodic x (fingering_periodic.py:13,
+    327-343); this mode is build-original, with the oracle extended alike."""
+    from fingering_dynamics_amd.lbm import Compute
+    from fingering_dynamics_amd.params import X_PERIODIC
+    H, W = 100, 400
+    mask, bits, links = rect_domain(H, W, 56, size=6, seed=1)
+    p = LbmParams.walled(H, W, shipped_walls=False, M=5.0, psi_wall=0.0, x_bc=X_PERIODIC, gx=1e-5,
+                         n_inv_cols=40, gamma=1.0027)
+    psi0 = np.where(np.arange(W)[None, :] < 40, 1.0, -1.0) * np.ones((H, 1))
+    cm = Compute(mask, bits, p, psi0=psi0)
+    o64 = Lbm64C.from_params(p, mask, links, psi0=psi0)
+    o32 = Lbm64C.from_params(p, mask, links, psi0=psi0, bits=32)
+    r0, s0, _ = cm.mass()
+    for t in range(500, 5001, 500):
+        cm.step(t - cm.t)
+        o64.step(t - o64.t)
+        o32.step(t - o32.t)
+        gp, fa, fb = cm.fields(), o64.fields(), o32.fields()
+        parts = []
+        for k, i in (("psi", 0), ("ux", 2), ("uy", 3)):
+            m = np.ones_like(mask) if k == "psi" else mask
+            g_inf, g_l2 = _errs(gp[i], fa[i], m)
+            r_inf, r_l2 = _errs(fb[i], fa[i], m)
+            parts.append("%s gpu %.2e/%.2e fp32-eval %.2e/%.2e" % (k, g_inf, g_l2, r_inf, r_l2))
+            assert g_inf <= TRAJ_FACTOR * r_inf + FLOOR_INF[k], (t, k, g_inf, r_inf)
+            assert g_l2 <= TRAJ_FACTOR * r_l2 + FLOOR_L2[k], (t, k, g_l2, r_l2)
+        g_rho, g_psi, bad = cm.mass()
+        a_rho, a_psi = o64.mass()
+        b_rho, b_psi = o32.mass()
+        nf = mask.sum()
+        assert bad == 0
+        # closed box: exact invariants of the FP64 update; FP32 arithmetic
+        # drifts them, the GPU by at most TRAJ_FACTOR x the FP32 evaluation
+        assert abs(a_rho - r0) < 1e-10 * abs(r0) and abs(a_psi - s0) < 1e-10 * nf, (t, a_rho, a_psi)
+        assert abs(g_rho - r0) <= TRAJ_FACTOR * abs(b_rho - r0) + MASS_FLOOR_REL * abs(r0), (t, g_rho, b_rho, r0)
+        assert abs(g_psi - s0) <= TRAJ_FACTOR * abs(b_psi - s0) + MASS_FLOOR_REL * nf, (t, g_psi, b_psi, s0)
+        print("t=%d" % t, "; ".join(parts), "; mass drift rho gpu %.1e fp32-eval %.1e (rel), psi gpu %.1e "
+              "fp32-eval %.1e (per node)" % ((g_rho - r0) / r0, (b_rho - r0) / r0, (g_psi - s0) / nf,
+                                             (b_psi - s0) / nf), flush=True)
+    cm.close()
