@@ -13,7 +13,11 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(2, 8, 12, 8, 16), (1, 5, 37, 32, 64), (2, 16, 16, 64, 128), (3, 9, 20, 16, 24)]
+SHAPES = [(2, 8, 12, 8, 16), (1, 5, 37, 32, 64), (2, 16, 16, 64, 128), (3, 9, 20, 16, 24),
+          # layers of the full-width network on the 256 x 128 coarse grid, batch 2:
+          # e0a, e0b, d0a (96 -> 32 after the skip concatenation), head, e1a, e3a
+          (2, 128, 256, 8, 32), (2, 128, 256, 32, 32), (2, 128, 256, 96, 32), (2, 128, 256, 32, 8),
+          (2, 64, 128, 32, 64), (2, 16, 32, 128, 128)]
 
 
 def _bf(t):

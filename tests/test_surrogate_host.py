@@ -224,5 +224,5 @@ def test_ensemble_domain_specs_and_square_beds():
     mask, links, p = ensemble.build_domain(spec)
     assert mask.shape == (spec["H"], spec["W"]) and links.shape == mask.shape
     assert p.H == spec["H"] and p.W == spec["W"] and p.M == spec["M"] and p.u0 == 0.001
-    assert p.wall_lo == 0 and p.wall_hi == spec["H"] - 1
+    assert p.y_bc == 0 and p.psi_visc_bound == 1.0      # periodic y, |psi| bound in tau_mix
     assert p.psi_wall == wetting.psi_wall_for_angle(180.0 - spec["theta"])
