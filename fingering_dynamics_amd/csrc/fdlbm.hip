@@ -577,7 +577,9 @@ __global__ __launch_bounds__(64 * WPB) void step_kernel(KParams k, KBufs b) {
 // shifted lane.  B needs A's output two columns either side of a node (one
 // for the f pull, one more for the g pulls of the psi stencil), so it is
 // valid on lanes 3..60: strips overlap by six columns of stage-A work.
-// For the chunk of output rows [y0, y1) stage A runs rows y0-2 ... y1+1.
+// For the chunk of output rows [y0, y1) stage A runs rows y0-2 ... y1+1,
+// upwards in even chunks and downwards in odd ones (B then lags two rows
+// behind in the march direction).
 //
 // Every node of either stage runs the single-iteration kernel's arithmetic
 // on the same values in the same order, so the result equals two
@@ -719,12 +721,19 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     __shared__ float fring_all[WPB * kRingRegion];
     float* const fring = fring_all + wave * kRingRegion + 1;
 
-    // ---- stage A prologue: psi_n+1 of rows y0-3, y0-2 and g of row y0-2
-    const int ys = y0 - 2;
+    // Even chunks march up (d = 1), odd chunks down (d = -1).  Stage A's
+    // rows beyond the chunk (two on either side) are read by both chunks
+    // beside a boundary; with alternating directions both read them at the
+    // same time (both first or both last), so the second read hits the L2.
+    const int d = (chunk & 1) ? -1 : 1;
+    const int ys = d > 0 ? y0 - 2 : y1 + 1;   // stage A's first row
+    const int yf = d > 0 ? y0 : y1 - 1;       // stage B's first row
+
+    // ---- stage A prologue: psi_n+1 of rows ys-d, ys and g of row ys
     float gc[9];
     PsiRow pb, pc;
     {
-        const int yb = ys - 1;
+        const int yb = ys - d;
         const unsigned fb = flags_of(k, b, ln, yb);
         const bool on = g_on(k, ln, yb, fb);
         pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
@@ -736,19 +745,20 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
         pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, gc);
         pc = lane_row(finish_g(k, ln, ys, flc, on, gc));
     }
-    unsigned fla = flags_of(k, b, ln, ys + 1);
+    unsigned fla = flags_of(k, b, ln, ys + d);
 
-    // ---- the hand-over: A's post-collision f of rows r .. r-3 in the LDS
-    // ring; its g of rows r (Ga), r-1 (Gb), r-2 (Gc: only directions 2, 5, 6
-    // are read) in registers; A's flags of rows r, r-1, r-2; A's side values
-    // P of rows r, r-1, r-2
+    // ---- the hand-over: A's post-collision f of rows r, r-d, r-2d, r-3d
+    // in the LDS ring (slot row & 3); its g of rows r (Ga), r-d (Gb), r-2d
+    // (Gc: only the three directions pulled from two rows back are read) in
+    // registers; A's flags of rows r, r-d, r-2d; A's side values P of the
+    // same rows
     float Ga[9], Gb[9], Gc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Ga[i] = Gb[i] = Gc[i] = 0.f;
     unsigned fr0 = kSolid, fr1 = kSolid, fr2 = kSolid;
     float pxa = 0.f, pya = 0.f, pxb = 0.f, pyb = 0.f, pxc = 0.f, pyc = 0.f;
-    // ---- stage B state: psi_n+2 of rows y-1 (qs), y (qc); g of row y (gq);
-    // inlet density of the previous row (walled inlet corners)
+    // ---- stage B state: psi_n+2 of rows y-d (qs), y (qc); g of row y (gq);
+    // inlet density of the row behind (walled inlet corners)
     PsiRow qs{}, qc{};
     float gq[9];
 #pragma unroll
@@ -757,21 +767,23 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
 
     const int nj = (y1 - y0) + 4;
     for (int j = 0; j < nj; ++j) {
-        const int r = ys + j;        // stage A row
+        const int r = ys + j * d;    // stage A row
         const bool arow = !(k.y_walls && (r < 0 || r >= k.H));
         const int rw = wrap(r, k.H);
         // ---------------- stage A, row r
         float Gn[9], pxn = 0.f, pyn = 0.f;
         {
-            const int ya = r + 1;
-            const unsigned fl2 = flags_of(k, b, ln, ya + 1);
+            const int ya = r + d;
+            const unsigned fl2 = flags_of(k, b, ln, ya + d);
             float ga[9], f[9];
             const bool ga_on = g_on(k, ln, ya, fla);
             pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
             const bool f_on = arow && ln.inside && !((flc & kSolid) && k.inert);
             pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, f_on, f);
             const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, ga_on, ga));
-            const float nb9[9] = {pc.c, pc.r, pa.c, pc.l, pb.c, pa.r, pa.l, pb.l, pb.r};
+            const PsiRow pn = d > 0 ? pa : pb;  // row r + 1
+            const PsiRow ps = d > 0 ? pb : pa;  // row r - 1
+            const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, ps.c, pn.r, pn.l, ps.l, ps.r};
             float dpx, dpy, lap;
             fdn::stencil(nb9, dpx, dpy, lap);
             if (arow) bc_f(k, b, src, rw, (int)ln.x, f);
@@ -810,32 +822,43 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
         pxa = pxn;
         pya = pyn;
         if (j < 2) continue;
-        // ---------------- stage B: psi_n+2 and g of row r-1
+        // ---------------- stage B: psi_n+2 and g of row r-d
         PsiRow qn;
         float gn[9];
         {
-            const int yq = r - 1;
+            const int yq = r - d;
             const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr1 & kSolid) && k.inert);
-            reg_pull(Gc, Gb, Ga, fr1 & kLinks, gn);
+            float lo[9], hi[9];  // A's rows yq - 1 and yq + 1
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                lo[i] = d > 0 ? Gc[i] : Ga[i];
+                hi[i] = d > 0 ? Ga[i] : Gc[i];
+            }
+            reg_pull(lo, Gb, hi, fr1 & kLinks, gn);
             qn = lane_row(finish_g(k, ln, yq, fr1, on, gn));
         }
         // walled inlet corners need the inlet density of the adjacent row:
-        // row y0-1 is the one stage B has not marched (top corner at y0 = H-1)
-        if (j == 3 && has_in && k.y_walls && y0 >= 1)
-            rho_prev = ring_inlet_rho(fring, y0 - 1, lane, fr2 & kLinks, b.uprof[y0 - 1], pxc);
+        // the row behind stage B's first row is one it has not marched
+        if (j == 3 && has_in && k.y_walls && yf - d >= 0 && yf - d < k.H)
+            rho_prev = ring_inlet_rho(fring, yf - d, lane, fr2 & kLinks, b.uprof[yf - d], pxc);
         if (j >= 4) {
-            // ------------ stage B: collision of row y = r-2
-            const int y = r - 2;
+            // ------------ stage B: collision of row y = r-2d
+            const int y = r - 2 * d;
             const unsigned fl = fr2;
             const bool solid = (fl & kSolid) != 0;
             float f[9];
             lds_pull(fring, y, lane, fl & kLinks, f);
             float rho_nb = rho_prev;
             if (has_in) {
-                if (k.y_walls && y == 0) rho_nb = ring_inlet_rho(fring, 1, lane, fr1 & kLinks, b.uprof[1], pxb);
+                // the corner met first (row 0 marching up, H-1 marching
+                // down) takes its neighbour row (ahead) from the ring
+                if (k.y_walls && y == (d > 0 ? 0 : k.H - 1))
+                    rho_nb = ring_inlet_rho(fring, y + d, lane, fr1 & kLinks, b.uprof[y + d], pxb);
                 rho_prev = fdn::inlet_rho(f, b.uprof[y], pxc);
             }
-            const float nb9[9] = {qc.c, qc.r, qn.c, qc.l, qs.c, qn.r, qn.l, qs.l, qs.r};
+            const PsiRow qa = d > 0 ? qn : qs;  // row y + 1
+            const PsiRow qb = d > 0 ? qs : qn;  // row y - 1
+            const float nb9[9] = {qc.c, qc.r, qa.c, qc.l, qb.c, qa.r, qa.l, qb.l, qb.r};
             float dpx, dpy, lap;
             fdn::stencil(nb9, dpx, dpy, lap);
             reg_bc_f(k, b, y, (int)ln.x, f, pxc, pyc, rho_nb);
@@ -1577,7 +1600,7 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
         h->wx = strips64 >= 16 ? 4 : 1;
         h->rows = (int)std::min<int64_t>(16, std::max<int64_t>(1, (strips64 * p->H + 4096) / 8192));
         h->fwx = 1;
-        h->frows = (int)std::min<int64_t>(12, std::max<int64_t>(1, (strips58 * p->H + 6144) / 12288));
+        h->frows = (int)std::min<int64_t>(8, std::max<int64_t>(1, (strips58 * p->H + 6144) / 12288));
     }
 
     const size_t Q = h->Q, H = p->H;

@@ -236,7 +236,7 @@ int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
  * registers, so each iteration reads and writes HBM once per two
  * iterations; depth 1 runs one launch per iteration.  The results are
  * identical bit for bit.  waves_x (1, 2 or 4) and rows set the two-iteration
- * kernel's launch shape (<= 0 keeps the current one; default 1 wave x 12 rows,
+ * kernel's launch shape (<= 0 keeps the current one; default 1 wave x 8 rows,
  * fewer rows on domains too small to fill the device).
  * Domains with H < 8, and walled domains whose walls are not the edge rows
  * (the rows 1 and H-2 of fingering.py:573), always run depth 1.  No reference counterpart (a
