@@ -137,9 +137,10 @@ def cpu_baseline(mask, bits, params, steps):
 
 def ensemble_bench(n, steps, warmup):
     """Aggregate MLUPS of n independent domains (the seeded many-domain
-    workload of fingering_dynamics_amd.ensemble.domain_spec) advanced by one
-    graph-replayed launch per iteration, against one launch per domain per
-    iteration on the same stream.  Prints one JSON line."""
+    workload of fingering_dynamics_amd.ensemble.domain_spec) advanced by
+    graph-replayed batched launches of two iterations each, against one
+    launch per domain per launch step on the same stream, and against
+    batched launches of one iteration.  Prints one JSON line."""
     from fingering_dynamics_amd.ensemble import Ensemble, domain_spec, build_domain
     from fingering_dynamics_amd.lbm import Compute
     t0 = time.perf_counter()
@@ -151,9 +152,14 @@ def ensemble_bench(n, steps, warmup):
     ens.synchronize()
     nodes = ens.nodes
     res = {}
-    for mode in ("graph", "eager", "sequential"):
+    fusion, fused = ens.fusion, ens.fused
+    for mode in ("graph", "eager", "sequential", "one_iteration_per_launch"):
         k = steps if mode != "sequential" else max(steps // 8, 10)
-        ens.use_graph(mode == "graph")
+        k += k & 1
+        ens.use_graph(mode != "eager")
+        if mode == "one_iteration_per_launch":
+            ens.set_fusion(1)
+            ens.step(4)
         t1 = time.perf_counter()
         (ens.step_sequential if mode == "sequential" else ens.step)(k)
         ens.synchronize()
@@ -163,14 +169,16 @@ def ensemble_bench(n, steps, warmup):
                      "mlups": round(nodes * k / wall / 1e6, 1)}
     bad = [c.mass()[2] for c in cs]
     out = {
-        "metric": "aggregate MLUPS of %d independent domains, one launch per iteration vs one per domain" % n,
+        "metric": "aggregate MLUPS of %d independent domains, batched launches vs one launch per domain" % n,
         "value": res["graph"]["mlups"], "unit": "MLUPS", "n_gpus": 1, "higher_is_better": True,
         "dtype": "fp32", "data": "synthetic: seeded squares/circles, sizes 256..1024 x 128..512",
         "config": {"workload": "many-domain ensemble", "domains": n, "nodes": nodes},
         "batched_graph": res["graph"], "batched_eager": res["eager"], "sequential": res["sequential"],
+        "batched_one_iteration_per_launch": res["one_iteration_per_launch"],
         "speedup_vs_sequential": round(res["graph"]["mlups"] / res["sequential"]["mlups"], 2),
+        "iterations_per_launch": 2 if fused else 1,
         "autotune_ms_per_step": {"%dx%d" % t: round(v, 4) for t, v in tune.items()},
-        "tile": "%dx%d" % ens.tile, "setup_s": round(setup_s, 1),
+        "shape": "%dx%d" % fusion[1:], "tile_one_iteration": "%dx%d" % ens.tile, "setup_s": round(setup_s, 1),
         "domains_non_finite": int(sum(1 for v in bad if v)), "iterations_run": int(cs[0].t),
     }
     ens.close()

@@ -64,6 +64,7 @@ EXPORTS = {
     "fd_ens_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_ens_step_sequential": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_ens_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "fd_ens_set_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fd_ens_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_ens_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "fd_ens_last_step_ms": (ctypes.c_double, [ctypes.c_void_p]),

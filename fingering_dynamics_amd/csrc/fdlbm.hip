@@ -696,9 +696,9 @@ __device__ __forceinline__ float collide_node(const KParams& k, float f[9], floa
     return mu;
 }
 
+// Two iterations of logical workgroup L of a launch over one domain.
 template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
-    const int L = banded_block();
+__device__ __forceinline__ void fused_march(const KParams& k, const KBufs& b, int L) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int chunk = L / k.nsb;
@@ -888,6 +888,11 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     }
 }
 
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
+    fused_march<WPB>(k, b, banded_block());
+}
+
 // One domain of a batched launch (an ensemble): its constants, flags,
 // inlet profile, both population and side-buffer sets, the parity of its
 // handle when the table was built, and its first logical workgroup.
@@ -925,6 +930,29 @@ __global__ __launch_bounds__(64 * WPB) void ens_kernel(const EnsDomain* __restri
     b.side_out = e.side[ps ^ 1];
     b.uprof = e.uprof;
     march_block<WPB, 0>(e.k, b, L - e.blk0);
+}
+
+// Two iterations of every domain of an ensemble in one launch: as
+// ens_kernel, over a table laid out for the two-iteration march.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void ens_fused_kernel(const EnsDomain* __restrict__ doms, int ndom,
+                                                             int parity) {
+    const int L = banded_block();
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (doms[mid].blk0 <= L) lo = mid; else hi = mid - 1;
+    }
+    const EnsDomain& e = doms[lo];
+    const int pc = e.cur0 ^ parity, ps = e.sc0 ^ parity;
+    KBufs b{};
+    b.src = e.pop[pc];
+    b.dst = e.pop[pc ^ 1];
+    b.flags = e.flags;
+    b.side_in = e.side[ps];
+    b.side_out = e.side[ps ^ 1];
+    b.uprof = e.uprof;
+    fused_march<WPB>(e.k, b, L - e.blk0);
 }
 
 // Initial state (fingering_periodic.py:90-121, or fingering.py:95-127 with
@@ -1368,23 +1396,33 @@ int64_t launches_for(const fd_handle* h, int64_t n) {
 // With the walls of fingering.py:573 on rows 1 and H-2, rows 0 and H-1 pull
 // across the edge through np.roll's wrap: those domains run one launch per
 // iteration.
-bool fusable(const fd_handle* h) {
-    const bool edge_walls = h->p.y_bc != FD_Y_WALLS || (h->p.wall_lo == 0 && h->p.wall_hi == h->p.H - 1);
-    return h->fuse == 2 && h->p.H >= 8 && edge_walls;
+// The domain's geometry allows the two-iteration kernel: H >= 8, and y
+// periodic or walled on the edge rows.
+bool geom_fusable(const fd_params& p) {
+    const bool edge_walls = p.y_bc != FD_Y_WALLS || (p.wall_lo == 0 && p.wall_hi == p.H - 1);
+    return p.H >= 8 && edge_walls;
 }
 
-int launch_fused(fd_handle* h, const KBufs& b) {
+bool fusable(const fd_handle* h) { return h->fuse == 2 && geom_fusable(h->p); }
+
+// Workgroups across x and chunks of the two-iteration kernel on a domain of
+// width W and height H with launch shape wx x rows.
+int fused_strip_groups(int W, int wx) { return ((W + kFuseCols - 1) / kFuseCols + wx - 1) / wx; }
+
+int launch_fused(fd_handle* h, const KBufs& b, hipStream_t stream = nullptr, int wx = 0, int rows = 0) {
+    if (wx <= 0) wx = h->fwx;
+    if (rows <= 0) rows = h->frows;
     KParams k = h->k;
-    k.rows = h->frows;
-    const int nstrips = (h->p.W + kFuseCols - 1) / kFuseCols;
-    k.nsb = (nstrips + h->fwx - 1) / h->fwx;
+    k.rows = rows;
+    k.nsb = fused_strip_groups(h->p.W, wx);
     k.sb0 = 0;
-    const int nchunks = (h->p.H + h->frows - 1) / h->frows;
-    const dim3 grid(k.nsb * nchunks), block(64 * h->fwx);
-    switch (h->fwx) {
-        case 1: hipLaunchKernelGGL((fused_kernel<1>), grid, block, 0, h->stream, k, b); break;
-        case 4: hipLaunchKernelGGL((fused_kernel<4>), grid, block, 0, h->stream, k, b); break;
-        default: hipLaunchKernelGGL((fused_kernel<2>), grid, block, 0, h->stream, k, b); break;
+    const int nchunks = (h->p.H + rows - 1) / rows;
+    const dim3 grid(k.nsb * nchunks), block(64 * wx);
+    if (!stream) stream = h->stream;
+    switch (wx) {
+        case 1: hipLaunchKernelGGL((fused_kernel<1>), grid, block, 0, stream, k, b); break;
+        case 4: hipLaunchKernelGGL((fused_kernel<4>), grid, block, 0, stream, k, b); break;
+        default: hipLaunchKernelGGL((fused_kernel<2>), grid, block, 0, stream, k, b); break;
     }
     FD_HIP(hipGetLastError());
     return FD_OK;
@@ -2205,13 +2243,15 @@ struct fd_ensemble {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evd = nullptr;
-    EnsDomain* table = nullptr;
-    int nblocks = 0;
-    int wx = 4, rows = 16;
+    EnsDomain* table = nullptr;      // single-iteration launch
+    EnsDomain* ftable = nullptr;     // two-iteration launch
+    int nblocks = 0, fnblocks = 0;
+    int wx = 4, rows = 16;           // single-iteration launch shape
+    int fuse = 2, fwx = 1, frows = 8;  // iterations per launch, two-iteration launch shape
     bool table_ok = false;
     int64_t flips = 0;               // launches since the table was built
     bool use_graph = true;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr, fexec = nullptr;
     int timed_launches = 0;
     bool have_timing = false;
     bool counted = false;            // holds a membership count on every handle
@@ -2219,13 +2259,25 @@ struct fd_ensemble {
 
 namespace {
 
+// Pairs of iterations go to the two-iteration kernel when every domain's
+// geometry allows it.
+bool ens_fused(const fd_ensemble* e) {
+    if (e->fuse != 2) return false;
+    for (const fd_handle* h : e->hs)
+        if (!geom_fusable(h->p)) return false;
+    return true;
+}
+
 int ens_build(fd_ensemble* e) {
-    if (e->exec) {
-        (void)hipGraphExecDestroy(e->exec);
-        e->exec = nullptr;
-    }
-    std::vector<EnsDomain> t(e->hs.size());
-    int64_t blk = 0;
+    for (hipGraphExec_t* x : {&e->exec, &e->fexec})
+        if (*x) {
+            (void)hipGraphExecDestroy(*x);
+            *x = nullptr;
+        }
+    std::vector<EnsDomain> t(e->hs.size()), ft;
+    int64_t blk = 0, fblk = 0;
+    const bool fused = ens_fused(e);
+    if (fused) ft.resize(e->hs.size());
     for (size_t i = 0; i < e->hs.size(); ++i) {
         fd_handle* h = e->hs[i];
         EnsDomain& d = t[i];
@@ -2245,19 +2297,42 @@ int ens_build(fd_ensemble* e) {
         d.blk0 = (int)blk;
         e->cur0[i] = h->cur;
         blk += (int64_t)d.k.nsb * ((h->p.H + e->rows - 1) / e->rows);
+        if (fused) {
+            EnsDomain& fd = ft[i];
+            fd = d;
+            fd.k.rows = e->frows;
+            fd.k.nsb = fused_strip_groups(h->p.W, e->fwx);
+            fd.blk0 = (int)fblk;
+            fblk += (int64_t)fd.k.nsb * ((h->p.H + e->frows - 1) / e->frows);
+        }
     }
-    if (blk >= ((int64_t)1 << 31)) return fail(FD_E_ARG, "ensemble too large for one launch");
+    if (blk >= ((int64_t)1 << 31) || fblk >= ((int64_t)1 << 31))
+        return fail(FD_E_ARG, "ensemble too large for one launch");
     FD_HIP(hipMemcpyAsync(e->table, t.data(), sizeof(EnsDomain) * t.size(), hipMemcpyHostToDevice, e->stream));
+    if (fused)
+        FD_HIP(hipMemcpyAsync(e->ftable, ft.data(), sizeof(EnsDomain) * ft.size(), hipMemcpyHostToDevice,
+                              e->stream));
     FD_HIP(hipStreamSynchronize(e->stream));
     e->nblocks = (int)blk;
+    e->fnblocks = fused ? (int)fblk : 0;
     e->flips = 0;
     e->table_ok = true;
     return FD_OK;
 }
 
-int ens_launch(fd_ensemble* e, int parity) {
-    const dim3 grid(e->nblocks), block(64 * e->wx);
+int ens_launch(fd_ensemble* e, int parity, bool fused = false) {
     const int n = (int)e->hs.size();
+    if (fused) {
+        const dim3 grid(e->fnblocks), block(64 * e->fwx);
+        switch (e->fwx) {
+            case 1: hipLaunchKernelGGL((ens_fused_kernel<1>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+            case 4: hipLaunchKernelGGL((ens_fused_kernel<4>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+            default: hipLaunchKernelGGL((ens_fused_kernel<2>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+        }
+        FD_HIP(hipGetLastError());
+        return FD_OK;
+    }
+    const dim3 grid(e->nblocks), block(64 * e->wx);
     switch (e->wx) {
         case 1: hipLaunchKernelGGL((ens_kernel<1>), grid, block, 0, e->stream, e->table, n, parity); break;
         case 2: hipLaunchKernelGGL((ens_kernel<2>), grid, block, 0, e->stream, e->table, n, parity); break;
@@ -2270,22 +2345,23 @@ int ens_launch(fd_ensemble* e, int parity) {
 
 // Two launches (parity 0, then 1) captured once and replayed: the buffer
 // sets alternate, so a pair returns every pointer to where it started.
-int ens_graph(fd_ensemble* e) {
-    if (e->exec) return FD_OK;
+int ens_graph(fd_ensemble* e, bool fused) {
+    hipGraphExec_t& exec = fused ? e->fexec : e->exec;
+    if (exec) return FD_OK;
     hipGraph_t g = nullptr;
     FD_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ens_launch(e, 0);
-    if (!rc) rc = ens_launch(e, 1);
+    int rc = ens_launch(e, 0, fused);
+    if (!rc) rc = ens_launch(e, 1, fused);
     const hipError_t ec = hipStreamEndCapture(e->stream, &g);
     if (rc) {
         if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (ec != hipSuccess) return fail(FD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-    const hipError_t ei = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
-        e->exec = nullptr;
+        exec = nullptr;
         return fail(FD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
     return FD_OK;
@@ -2315,44 +2391,54 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
             FD_HIP(hipStreamWaitEvent(e->stream, h->evs, 0));
         }
         FD_HIP(hipEventRecord(e->ev0, e->stream));
+        // iterations per launch: pairs go to the two-iteration kernel
+        const int per = ens_fused(e) ? 2 : 1;
         if (sequential) {
-            // the baseline: one launch per domain per iteration, one stream
-            for (int64_t i = 0; i < nl; ++i)
+            // the baseline: one launch per domain per launch step, one stream
+            for (int64_t i = 0; i < nl;) {
+                const int it = (per == 2 && nl - i >= 2) ? 2 : 1;
                 for (fd_handle* h : e->hs) {
-                    KBufs b{};
-                    b.src = h->pop[h->cur];
-                    b.dst = h->pop[1 - h->cur];
-                    b.flags = h->flags;
-                    b.side_in = h->side[h->sc];
-                    b.side_out = h->side[1 - h->sc];
-                    b.uprof = h->uprof;
-                    const int wx = h->wx, rows = h->rows;
-                    h->wx = e->wx;
-                    h->rows = e->rows;
-                    rc = launch_step(h, 0, b, 0, -1, e->stream);
-                    h->wx = wx;
-                    h->rows = rows;
+                    const KBufs b = step_bufs(h, h->cur, h->sc);
+                    if (it == 2) {
+                        rc = launch_fused(h, b, e->stream, e->fwx, e->frows);
+                    } else {
+                        const int wx = h->wx, rows = h->rows;
+                        h->wx = e->wx;
+                        h->rows = e->rows;
+                        rc = launch_step(h, 0, b, 0, -1, e->stream);
+                        h->wx = wx;
+                        h->rows = rows;
+                    }
                     if (rc) return rc;
                     h->cur ^= 1;
                     h->sc ^= 1;
                 }
-            e->flips += nl;
-        } else {
-            int64_t i = 0;
-            if ((e->flips & 1) != 0) {
-                if ((rc = ens_launch(e, 1))) return rc;
-                ++i;
+                i += it;
                 ++e->flips;
             }
-            if (e->use_graph && nl - i >= 2) {
-                if ((rc = ens_graph(e))) return rc;
-                for (; nl - i >= 2; i += 2, e->flips += 2) FD_HIP(hipGraphLaunch(e->exec, e->stream));
+        } else {
+            // every launch, of one iteration or two, flips the buffer parity
+            int64_t i = 0, launches = 0;
+            auto one = [&](int it) {
+                const int r = ens_launch(e, (int)(e->flips & 1), it == 2);
+                if (!r) {
+                    i += it;
+                    ++e->flips;
+                    ++launches;
+                }
+                return r;
+            };
+            if ((e->flips & 1) != 0 && (rc = one(per == 2 && nl >= 2 ? 2 : 1))) return rc;
+            if (e->use_graph && nl - i >= 2 * per) {
+                if ((rc = ens_graph(e, per == 2))) return rc;
+                for (; nl - i >= 2 * per; i += 2 * per, e->flips += 2, launches += 2)
+                    FD_HIP(hipGraphLaunch(per == 2 ? e->fexec : e->exec, e->stream));
             }
-            for (; i < nl; ++i, ++e->flips)
-                if ((rc = ens_launch(e, (int)(e->flips & 1)))) return rc;
+            while (i < nl)
+                if ((rc = one(per == 2 && nl - i >= 2 ? 2 : 1))) return rc;
             for (fd_handle* h : e->hs) {
-                h->cur ^= (int)(nl & 1);
-                h->sc ^= (int)(nl & 1);
+                h->cur ^= (int)(launches & 1);
+                h->sc ^= (int)(launches & 1);
             }
         }
         FD_HIP(hipEventRecord(e->ev1, e->stream));
@@ -2404,6 +2490,7 @@ int fd_ens_create(fd_handle* const* hs, int32_t n, fd_ensemble** out) {
     FD_HIP_E(hipEventCreate(&e->ev1));
     FD_HIP_E(hipEventCreateWithFlags(&e->evd, hipEventDisableTiming));
     FD_HIP_E(hipMalloc(&e->table, sizeof(EnsDomain) * (size_t)n));
+    FD_HIP_E(hipMalloc(&e->ftable, sizeof(EnsDomain) * (size_t)n));
 #undef FD_HIP_E
     for (fd_handle* h : e->hs) ++h->ens_refs;
     e->counted = true;
@@ -2422,6 +2509,17 @@ int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows) {
     if (rows < 1) return fail(FD_E_ARG, "rows must be >= 1");
     e->wx = waves_x;
     e->rows = rows;
+    e->table_ok = false;
+    return FD_OK;
+}
+
+int fd_ens_set_fusion(fd_ensemble* e, int32_t depth, int32_t waves_x, int32_t rows) {
+    if (!e) return fail(FD_E_ARG, "ensemble is NULL");
+    if (depth != 1 && depth != 2) return fail(FD_E_ARG, "depth must be 1 or 2");
+    if (waves_x > 0 && waves_x != 1 && waves_x != 2 && waves_x != 4) return fail(FD_E_ARG, "waves_x must be 1, 2 or 4");
+    e->fuse = depth;
+    if (waves_x > 0) e->fwx = waves_x;
+    if (rows > 0) e->frows = rows;
     e->table_ok = false;
     return FD_OK;
 }
@@ -2454,7 +2552,9 @@ void fd_ens_destroy(fd_ensemble* e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->fexec) (void)hipGraphExecDestroy(e->fexec);
     if (e->table) (void)hipFree(e->table);
+    if (e->ftable) (void)hipFree(e->ftable);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->evd) (void)hipEventDestroy(e->evd);

@@ -1,14 +1,15 @@
-"""Ensembles of independent domains advanced by one launch per iteration,
+"""Ensembles of independent domains advanced together by batched launches,
 and the seeded generator of the many-domain workload.
 
 The reference runs one domain per process (fingering_periodic.py:454-479).
 Here any number of ``Compute`` solvers on one GPU - different sizes,
 viscosity ratios, wall values, obstacles and boundary conditions - are
 batched: one kernel launch walks a device table of the domains and advances
-all of them by one iteration, and step loops are replayed from a captured
-HIP graph.  ``step_sequential`` runs the same iterations as one launch per
-domain on the same stream, the baseline the batched launch is measured
-against; both give bit-identical states.
+all of them by two iterations (the two-iteration kernel of fd_step, when
+every domain's geometry allows it) or by one, and step loops are replayed
+from a captured HIP graph.  ``step_sequential`` runs the same iterations as
+one launch per domain on the same stream, the baseline the batched launch
+is measured against; both give bit-identical states.
 
 ``autotune`` picks the launch shape by timing a few real iterations per
 candidate; the results do not depend on the shape (every node's update is
@@ -33,9 +34,10 @@ import math
 import numpy as np
 
 from . import _native, geometry, wetting
-from .params import LbmParams, PROFILE_CONST, OUTLET_PRESSURE
+from .params import LbmParams, PROFILE_CONST, OUTLET_PRESSURE, Y_WALLS
 
 TILE_CANDIDATES = ((4, 16), (4, 8), (4, 32), (8, 16), (2, 16), (1, 16))
+FUSED_CANDIDATES = ((1, 8), (1, 6), (1, 12), (1, 16), (2, 8), (4, 8))
 
 
 class Ensemble:
@@ -51,6 +53,7 @@ class Ensemble:
         _native.check(self._lib.fd_ens_create(arr, len(self.computes), _native.ctypes.byref(h)))
         self._h = h
         self.tile = (4, 16)
+        self.fusion = (2, 1, 8)
         # the device table points at every member's buffers: a member may not
         # be closed while the ensemble exists (Compute.close raises)
         for c in self.computes:
@@ -61,7 +64,7 @@ class Ensemble:
         return sum(c.H * c.W for c in self.computes)
 
     def step(self, n=1):
-        """n iterations of every domain, one launch per iteration (asynchronous)."""
+        """n iterations of every domain, batched launches (asynchronous)."""
         _native.check(self._lib.fd_ens_step(self._h, int(n)))
         self._invalidate()
 
@@ -75,8 +78,24 @@ class Ensemble:
             c._cache = None
 
     def set_tile(self, waves_x, rows):
+        """Launch shape of the single-iteration batched kernel."""
         _native.check(self._lib.fd_ens_set_tile(self._h, int(waves_x), int(rows)))
         self.tile = (int(waves_x), int(rows))
+
+    def set_fusion(self, depth, waves_x=0, rows=0):
+        """Iterations per batched launch (2 or 1) and the two-iteration
+        kernel's launch shape (<= 0 keeps the current one)."""
+        _native.check(self._lib.fd_ens_set_fusion(self._h, int(depth), int(waves_x), int(rows)))
+        wx = int(waves_x) if waves_x > 0 else self.fusion[1]
+        r = int(rows) if rows > 0 else self.fusion[2]
+        self.fusion = (int(depth), wx, r)
+
+    @property
+    def fused(self):
+        """True when pairs of iterations run as two-iteration launches."""
+        return self.fusion[0] == 2 and all(
+            c.H >= 8 and (c.params.y_bc != Y_WALLS or (c.params.wall_lo == 0 and c.params.wall_hi == c.H - 1))
+            for c in self.computes)
 
     def use_graph(self, on=True):
         _native.check(self._lib.fd_ens_use_graph(self._h, 1 if on else 0))
@@ -87,17 +106,28 @@ class Ensemble:
     def last_step_ms(self):
         return float(self._lib.fd_ens_last_step_ms(self._h))
 
-    def autotune(self, candidates=TILE_CANDIDATES, steps=20):
-        """Time ``steps`` real iterations per launch shape and keep the
-        fastest.  Returns {tile: ms per iteration}."""
+    def autotune(self, candidates=None, steps=20):
+        """Time ``steps`` real iterations per launch shape of the kernel that
+        runs (two-iteration when ``fused``) and keep the fastest.  Returns
+        {shape: ms per iteration}."""
+        fused = self.fused
+        if candidates is None:
+            candidates = FUSED_CANDIDATES if fused else TILE_CANDIDATES
+        steps += steps & 1
         times = {}
         for wx, rows in candidates:
-            self.set_tile(wx, rows)
-            self.step(2)                         # table rebuild and graph capture
+            if fused:
+                self.set_fusion(2, wx, rows)
+            else:
+                self.set_tile(wx, rows)
+            self.step(4)                         # table rebuild and graph capture
             self.step(steps)
             times[(wx, rows)] = self.last_step_ms()
         best = min(times, key=times.get)
-        self.set_tile(*best)
+        if fused:
+            self.set_fusion(2, *best)
+        else:
+            self.set_tile(*best)
         return times
 
     def close(self):

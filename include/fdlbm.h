@@ -292,8 +292,19 @@ int fd_ens_step(fd_ensemble* e, int32_t nsteps);
  * against.  Results are identical to fd_ens_step. */
 int fd_ens_step_sequential(fd_ensemble* e, int32_t nsteps);
 
-/* Launch shape of the batched kernel, as fd_set_tile (default 4 x 16). */
+/* Launch shape of the batched single-iteration kernel, as fd_set_tile
+ * (default 4 x 16). */
 int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows);
+
+/* Iterations per batched launch, as fd_set_fusion: depth 2 (the default)
+ * advances pairs of iterations of every domain with one launch of the
+ * two-iteration kernel, depth 1 one iteration per launch.  waves_x (1, 2 or
+ * 4) and rows set the two-iteration launch shape (<= 0 keeps the current
+ * one; default 1 x 8).  Pairs run batched only when every domain's geometry
+ * allows the two-iteration kernel (fd_set_fusion); otherwise every launch
+ * is of one iteration.  Results are identical bit for bit in every case.
+ * No reference counterpart (the reference runs one domain per process). */
+int fd_ens_set_fusion(fd_ensemble* e, int32_t depth, int32_t waves_x, int32_t rows);
 
 /* 1: replay step loops from a captured graph (default); 0: launch eagerly. */
 int fd_ens_use_graph(fd_ensemble* e, int32_t on);

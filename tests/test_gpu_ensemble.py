@@ -1,5 +1,7 @@
-"""Batched ensembles: one launch per iteration over many domains must give
-exactly the states the domains reach when stepped one by one."""
+"""Batched ensembles: batched launches over many domains (two iterations
+per launch by default, one when asked or when a member's geometry does not
+allow it) must give exactly the states the domains reach when stepped one
+by one."""
 import numpy as np
 import pytest
 
@@ -57,11 +59,14 @@ def _same(a, b):
     return True
 
 
-def test_batched_equals_individual():
+@pytest.mark.parametrize("depth", [2, 1])
+def test_batched_equals_individual(depth):
     from fingering_dynamics_amd.ensemble import Ensemble
     ref = _solvers()
     cs = _solvers()
     ens = Ensemble(cs)
+    ens.set_fusion(depth)
+    assert ens.fused == (depth == 2)
     done = 0
     # graph replays (even chunks), odd chunks, eager launches, per-handle
     # reads between chunks, and a sequential stretch
@@ -99,8 +104,14 @@ def test_tile_independent_and_mass():
         assert bad == 0 and np.isfinite(s_rho)
     times = ea.autotune(candidates=((4, 16), (2, 8)), steps=4)
     assert set(times) == {(4, 16), (2, 8)} and all(t > 0 for t in times.values())
-    eb.step(10)                      # the autotuner's 2 x 6 trial iterations
-    eb.step(2)
+    assert ea.fusion[1:] in times
+    eb.step(a[0].t - b[0].t)         # the autotuner's trial iterations
+    assert _same(_state(a), _state(b))
+    # the same with one iteration per launch: the single-iteration shapes
+    ea.set_fusion(1)
+    times = ea.autotune(candidates=((4, 16), (1, 5)), steps=3)
+    assert ea.tile in times
+    eb.step(a[0].t - b[0].t)
     assert _same(_state(a), _state(b))
     ea.close()
     eb.close()
@@ -122,5 +133,41 @@ def test_misuse_raises():
     e.step(2)
     assert cs[0].t == cs[1].t == 5
     e.close()
+    for c in cs:
+        c.close()
+
+
+def test_member_without_two_iteration_geometry():
+    """A member with the reference's wall rows 1 and H-2 (fingering.py:573)
+    or H < 8 keeps the batch at one iteration per launch; still exact."""
+    from fingering_dynamics_amd.ensemble import Ensemble
+    from fingering_dynamics_amd.lbm import Compute
+    specs = _specs()[:2]
+    H, W = 34, 70
+    m, b, _ = rect_domain(H, W, 6, size=4, seed=5, x_min=15)
+    specs.append((m, b, LbmParams.walled(H, W, M=10.0, psi_wall=-0.5, gamma=1.0027)))
+    assert specs[-1][2].wall_lo == 1
+    m, b, _ = rect_domain(6, 80, 2, size=2, seed=6, x_min=15)
+    specs.append((m, b, LbmParams.periodic(6, 80, M=4.0, psi_wall=0.1, n_inv_cols=10, profile=PROFILE_CONST)))
+    for drop in (None, 2, 3):
+        use = [sp for i, sp in enumerate(specs) if i != drop]
+        cs = [Compute(*sp) for sp in use]
+        ref = [Compute(*sp) for sp in use]
+        ens = Ensemble(cs)
+        assert not ens.fused
+        for n in (3, 8):
+            ens.step(n)
+            ens.step_sequential(1)
+            for r in ref:
+                r.step(n + 1)
+            assert _same(_state(cs), _state(ref)), (drop, n)
+        ens.close()
+        for c in cs + ref:
+            c.close()
+    # without both: pairs run batched
+    cs = [Compute(*sp) for sp in specs[:2]]
+    ens = Ensemble(cs)
+    assert ens.fused
+    ens.close()
     for c in cs:
         c.close()
