@@ -150,6 +150,153 @@ __global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, 
     }
 }
 
+// Forward convolution with the input staged in LDS (Cin a multiple of 32).
+// A workgroup owns an output block of WR*2 image rows x 32 columns of one
+// image and WN*WC*16 output channels (WN = 4 / WR).  Per chunk of 32 input
+// channels it stages the block's input halo - (WR*2 + 2) rows x 34 columns
+// x 32 channels, zeros outside the image, the ReLU mask of xm applied on
+// load - in LDS, each pixel padded to 40 channels (80 bytes: the 16 lanes
+// of one MFMA row read 16 different bank groups), and every tap's A
+// fragments are 16-byte LDS reads at the shifted pixel.  Wave (wr, wn)
+// computes rows 2wr, 2wr+1 (four 16-pixel tiles) x WC channel tiles; its B
+// fragments (weights) are 16-byte global loads that the L1 shares between
+// the waves and workgroups using the same channels.  The next chunk's halo
+// is loaded into registers while the current one is consumed (two LDS
+// buffers, one barrier per chunk).
+constexpr int kHaloC = 40;   // padded channels per staged pixel
+constexpr int kHaloW = 34;   // 32 columns + one either side
+
+template <int WR, int WC, bool MASK>
+__global__ __launch_bounds__(256) void conv_fwd_lds(const uint16_t* __restrict__ x, const uint16_t* __restrict__ xm,
+                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                    float* __restrict__ y32, uint16_t* __restrict__ y16, int N, int H,
+                                                    int W, int Cin, int Cout, int relu) {
+    constexpr int WN = 4 / WR;
+    constexpr int BR = 2 * WR;                    // output rows of the block
+    constexpr int HR = BR + 2;                    // halo rows
+    constexpr int HP = HR * kHaloW;               // halo pixels
+    constexpr int PIECES = HP * 4;                // 16-byte pieces (8 channels) per chunk
+    constexpr int PPT = (PIECES + 255) / 256;     // pieces per thread
+    __shared__ __align__(16) uint16_t halo[2][HP * kHaloC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave % WR, wn = wave / WR;
+    const int wblocks = (W + 31) / 32, hblocks = (H + BR - 1) / BR;
+    int b = blockIdx.x;
+    const int bw = b % wblocks;
+    b /= wblocks;
+    const int bh = b % hblocks;
+    const int n = b / hblocks;
+    const int h0 = bh * BR, w0 = bw * 32;
+    const int co0 = blockIdx.y * (WN * WC * 16) + wn * (WC * 16);
+    const int K = 9 * Cin;
+    const int64_t img = (int64_t)n * H;
+
+    // staging: piece q = pixel q / 4 of the halo, channels 8 * (q % 4)
+    auto fetch = [&](int ci0, u4v v[PPT]) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int q = threadIdx.x + 256 * j;
+            v[j] = u4v{0u, 0u, 0u, 0u};
+            if (q < PIECES) {
+                const int px = q >> 2, g = q & 3;
+                const int hh = h0 - 1 + px / kHaloW, ww = w0 - 1 + px % kHaloW;
+                if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
+                    const int64_t o = ((img + hh) * W + ww) * Cin + ci0 + 8 * g;
+                    v[j] = *reinterpret_cast<const u4v*>(x + o);
+                    if (MASK) v[j] = relu_keep(v[j], *reinterpret_cast<const u4v*>(xm + o));
+                }
+            }
+        }
+    };
+    auto stage = [&](int buf, const u4v v[PPT]) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int q = threadIdx.x + 256 * j;
+            if (q < PIECES)
+                *reinterpret_cast<u4v*>(&halo[buf][(q >> 2) * kHaloC + 8 * (q & 3)]) = v[j];
+        }
+    };
+
+    v4f acc[4][WC];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < WC; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    u4v nxt[PPT];
+    fetch(0, nxt);
+    stage(0, nxt);
+    __syncthreads();
+    const int nchunks = Cin / 32;
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) fetch(32 * (c + 1), nxt);
+        const uint16_t* hb = halo[buf];
+        // fragments of tap t + 1 are loaded while tap t's MFMAs run
+        auto load_b = [&](int tap, u4v bv[WC]) {
+            const int k0 = tap * Cin + 32 * c + 8 * (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < WC; ++t) {
+                const int co = co0 + 16 * t + (lane & 15);
+                bv[t] = co < Cout ? *reinterpret_cast<const u4v*>(w + (int64_t)co * K + k0) : u4v{0u, 0u, 0u, 0u};
+            }
+        };
+        auto load_a = [&](int tap, u4v av[4]) {
+            const int dh = tap / 3, dw = tap % 3;     // halo offsets (0..2)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                // tile m: block row 2 wr + m / 2, columns 16 (m % 2) .. + 15
+                const int hr = 2 * wr + (m >> 1) + dh;
+                const int hc = 16 * (m & 1) + (lane & 15) + dw;
+                av[m] = *reinterpret_cast<const u4v*>(&hb[(hr * kHaloW + hc) * kHaloC + 8 * (lane >> 4)]);
+            }
+        };
+        u4v bv[WC], av[4], bn[WC], an[4];
+        load_b(0, bv);
+        load_a(0, av);
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            if (tap < 8) {
+                load_b(tap + 1, bn);
+                load_a(tap + 1, an);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < WC; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf8(av[m]), bf8(bv[t]), acc[m][t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < WC; ++t) bv[t] = bn[t];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) av[m] = an[m];
+        }
+        if (c + 1 < nchunks) {
+            stage(buf ^ 1, nxt);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < WC; ++t) {
+        const int co = co0 + 16 * t + (lane & 15);
+        if (co >= Cout) continue;
+        const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int h = h0 + 2 * wr + (m >> 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int wc = w0 + 16 * (m & 1) + 4 * (lane >> 4) + r;
+                if (h >= H || wc >= W) continue;
+                const int64_t p = (img + h) * W + wc;
+                float v = acc[m][t][r] + bb;
+                if (relu) v = fmaxf(v, 0.f);
+                if (y32) y32[p * Cout + co] = v;
+                if (y16) y16[p * Cout + co] = to_bf16(v);
+            }
+        }
+    }
+}
+
 // wt[ci][tap][co] = w[co][8 - tap][ci]
 __global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt, int Cin,
                                                     int Cout) {
@@ -456,6 +603,27 @@ template <bool MASK>
 void launch_fwd(const uint16_t* x, const uint16_t* xm, const uint16_t* w, const float* bias, float* y32, uint16_t* y16,
                 int N, int H, int W, int Cin, int Cout, int relu, hipStream_t st) {
     const int64_t P = (int64_t)N * H * W;
+    if (Cin % 32 == 0 && Cin >= 64 && Cout >= 32) {
+        // input staged in LDS: blocks of 4 rows x 32 columns x 128 channels
+        // (WR 2, WC 4) when Cout is a multiple of 128, 8 rows x 32 (WR 4,
+        // WC 2) for Cout < 64, else 8 rows x 64 (WR 4, WC 4); the measured
+        // per-layer times favour the direct kernel below when Cin is 32
+        const unsigned wb = (unsigned)((W + 31) / 32);
+        if (Cout < 64) {
+            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N, (unsigned)((Cout + 31) / 32));
+            hipLaunchKernelGGL((conv_fwd_lds<4, 2, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
+                               Cin, Cout, relu);
+        } else if (Cout % 128 == 0) {
+            const dim3 grid(wb * (unsigned)((H + 3) / 4) * (unsigned)N, (unsigned)((Cout + 127) / 128));
+            hipLaunchKernelGGL((conv_fwd_lds<2, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
+                               Cin, Cout, relu);
+        } else {
+            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N, (unsigned)((Cout + 63) / 64));
+            hipLaunchKernelGGL((conv_fwd_lds<4, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
+                               Cin, Cout, relu);
+        }
+        return;
+    }
     // wave tiles: 64 px x 64 co, 128 px x 32 co, or 128 px x 16 co
     if (Cout >= 64) {
         const dim3 grid((unsigned)((P + 255) / 256), (unsigned)((Cout + 63) / 64));
