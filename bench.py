@@ -249,7 +249,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="iterations of the CPU baseline sample (0 skips it)")
     ap.add_argument("--tile", default="", help="launch shape WX,ROWS of the one-iteration kernel")
-    ap.add_argument("--fusion", type=int, default=2, help="iterations per launch (1 or 2; default 2)")
+    ap.add_argument("--fusion", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="iterations per launch (1, 2 or 3; default 0: the library's choice for the domain)")
     ap.add_argument("--ftile", default="", help="launch shape WX,ROWS of the two-iteration kernel")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run the plain-copy HBM probe (1 GiB copies) and print its bandwidth")
@@ -288,7 +289,9 @@ def main():
         wx, rows = (int(v) for v in args.tile.split(","))
         cm.set_tile(wx, rows)
     fwx, frows = (int(v) for v in args.ftile.split(",")) if args.ftile else (0, 0)
-    cm.set_fusion(args.fusion, fwx, frows)
+    if args.fusion or args.ftile:
+        cm.set_fusion(args.fusion or cm.fusion[0], fwx, frows)
+    fusion = cm.fusion
     cm.step(args.warmup)
     cm.synchronize()
     t0 = time.perf_counter()
@@ -303,10 +306,11 @@ def main():
 
     ms_per_step = dt * 1e3 / args.steps
     mlups = H * W * args.steps / dt / 1e6
-    # the dominant kernel: fused_kernel (two iterations per launch) or
-    # step_kernel (one); kernel_ms is the device time per iteration
-    per_launch = 2 if args.fusion == 2 else 1
-    kernel = "fused_kernel" if per_launch == 2 else "step_kernel"
+    # the dominant kernel: fused3_kernel (three iterations per launch),
+    # fused_kernel (two) or step_kernel (one); kernel_ms is the device time
+    # per iteration
+    per_launch = fusion[0]
+    kernel = {3: "fused3_kernel", 2: "fused_kernel", 1: "step_kernel"}[per_launch]
     algo_bytes = per_launch * BYTES_PER_FLUID_UPDATE * n_fluid
     launch_ms = kernel_ms * per_launch
     achieved = algo_bytes / (launch_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
@@ -352,6 +356,7 @@ def main():
         },
         "mflups": round(n_fluid * args.steps / dt / 1e6, 1),
         "mass_check": {"sum_rho": s_rho, "sum_psi": s_psi, "non_finite": bad, "min_tau_minus_half": tau_margin},
+        "launch": {"iterations_per_launch": fusion[0], "waves_per_workgroup": fusion[1], "rows_per_wave": fusion[2]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -35,6 +35,8 @@ EXPORTS = {
     "fd_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fd_init_fields": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "fd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fd_get_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32)]),
     "fd_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fd_steps_done": (ctypes.c_int64, [ctypes.c_void_p]),
     "fd_solid_inert": (ctypes.c_int, [ctypes.c_void_p]),

@@ -116,6 +116,20 @@ __device__ __forceinline__ rsrc_t make_rsrc(const float* p, unsigned bytes) {
 // wrapped in x when x is periodic (np.roll semantics).  For Zou-He x the
 // column -1 / W reads of the first / last column land inside the
 // allocation (planes >= 1, padding columns) and are rewritten by Zou-He.
+// (links bit b set) ? a : c, as a sign-extended one-bit field extract and
+// a bitfield insert: two VALU operations and no compare into VCC (the
+// compiler turns the same expression written in C into and + compare +
+// select).
+__device__ __forceinline__ unsigned sel_bit(unsigned links, int b, unsigned a, unsigned c) {
+    unsigned m, r;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(links), "i"(b));
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float sel_bit(unsigned links, int b, float a, float c) {
+    return __builtin_bit_cast(float, sel_bit(links, b, __builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, c)));
+}
+
 struct PullOff {
     unsigned v[9];
     unsigned xb;  // the lane's own column in bytes
@@ -151,7 +165,7 @@ __device__ __forceinline__ void pull(rsrc_t src, const KParams& k, int P0, const
         if (i > 0) {
             // own node, plane opp(i), row y, relative to this load's scalar offset
             const unsigned d = (unsigned)(fdn::OPP[i] * k.Q + r.c - row) * 4u;
-            vo = ((links >> (i - 1)) & 1u) ? po.xb + d : vo;
+            vo = sel_bit(links, i - 1, po.xb + d, vo);
         }
         v[i] = __builtin_bit_cast(float,
                                   __builtin_amdgcn_raw_buffer_load_b32(src, on ? vo : kOOB, (row + gq) * 4, 0));
@@ -620,7 +634,7 @@ __device__ __forceinline__ void reg_pull(const float lo[9], const float mid[9], 
     for (int i = 0; i < 9; ++i) {
         const float* row = fdn::EY[i] > 0 ? lo : (fdn::EY[i] < 0 ? hi : mid);
         float x = lane_pull(row[i], fdn::EX[i]);
-        if (i > 0) x = ((links >> (i - 1)) & 1u) ? mid[fdn::OPP[i]] : x;
+        if (i > 0) x = sel_bit(links, i - 1, mid[fdn::OPP[i]], x);
         v[i] = x;
     }
 }
@@ -640,7 +654,7 @@ __device__ __forceinline__ void lds_pull(const float* ring, int y, int lane, uns
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         int idx = ((y - fdn::EY[i]) & 3) * kRingSlot + i * 64 + lane - fdn::EX[i];
-        if (i > 0) idx = ((links >> (i - 1)) & 1u) ? (y & 3) * kRingSlot + fdn::OPP[i] * 64 + lane : idx;
+        if (i > 0) idx = (int)sel_bit(links, i - 1, (unsigned)((y & 3) * kRingSlot + fdn::OPP[i] * 64 + lane), (unsigned)idx);
         v[i] = ring[idx];
     }
 }
@@ -891,6 +905,285 @@ __device__ __forceinline__ void fused_march(const KParams& k, const KBufs& b, in
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
     fused_march<WPB>(k, b, banded_block());
+}
+
+// ---------------------------------------------------- three-iteration march
+// One launch advances the state by three iterations, S_n -> S_n+3: stage A
+// as above; stage B (two rows behind A) keeps S_n+2 on chip - g in a second
+// register ring, f in a second LDS ring - instead of storing it; stage C
+// (two rows behind B) pulls S_n+2 from them, collides and stores S_n+3.
+// Each stage needs two columns more of the stage before it, so the 64
+// lanes hold columns x0 - 5 ... x0 + 58 and stage C's 54 output columns
+// are lanes 5..58.  For the chunk [y0, y1) stage A runs rows y0-4 ... y1+3
+// and stage B rows y0-2 ... y1+1, which may lie beyond a walled domain's
+// edge rows (ghost rows: psi_wall, no populations) or wrap around a
+// periodic one.  The arithmetic of every node is the single-iteration
+// kernel's, in the same order, so the result equals three single-iteration
+// launches bit for bit.
+constexpr int kFuse3Cols = 54;
+
+template <int WPB>
+__device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, int L) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = L / k.nsb;
+    const int strip = (L - chunk * k.nsb) * WPB + wave;
+    const int xc0 = strip * kFuse3Cols;
+    if (xc0 >= k.W) return;
+    const int xa = xc0 - 5 + lane;
+    const Lane ln = lane_of(k, xa);
+    const bool cout = lane >= 5 && lane < 5 + kFuse3Cols && xa < k.W;   // stage C stores this lane
+    const bool zx = !k.x_periodic;
+    const bool has_in = zx && xc0 == 0;                                  // column 0 among the outputs
+    const int y0 = chunk * k.rows;
+    const int y1 = min(y0 + k.rows, k.H);
+
+    const unsigned bytes = 18u * (unsigned)k.Q * 4u;
+    const rsrc_t src = make_rsrc(b.src, bytes);
+    const rsrc_t dst = make_rsrc(b.dst, bytes);
+    const PullOff po = pull_offsets(k, ln.x);
+    __shared__ float rings_all[WPB * 2 * kRingRegion];
+    float* const fring = rings_all + wave * 2 * kRingRegion + 1;    // A -> B
+    float* const fring2 = fring + kRingRegion;                       // B -> C
+
+    const int d = (chunk & 1) ? -1 : 1;
+    const int ys = d > 0 ? y0 - 4 : y1 + 3;   // stage A's first row
+    const int yfb = ys + 2 * d;               // stage B's first row
+    const int yfc = ys + 4 * d;               // stage C's first row
+
+    // ---- stage A prologue: psi_n+1 of rows ys-d, ys and g of row ys
+    float gc[9];
+    PsiRow pb, pc;
+    {
+        const int yb = ys - d;
+        const unsigned fb = flags_of(k, b, ln, yb);
+        const bool on = g_on(k, ln, yb, fb);
+        pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
+        pb = lane_row(finish_g(k, ln, yb, fb, on, gc));
+    }
+    unsigned flc = flags_of(k, b, ln, ys);
+    {
+        const bool on = g_on(k, ln, ys, flc);
+        pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, gc);
+        pc = lane_row(finish_g(k, ln, ys, flc, on, gc));
+    }
+    unsigned fla = flags_of(k, b, ln, ys + d);
+
+    // A -> B: g of A's rows r (Ga), r-d (Gb), r-2d (Gc); side values P of
+    // the same rows.  Flags of rows r, r-d, ..., r-4d (ghost rows solid).
+    float Ga[9], Gb[9], Gc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ga[i] = Gb[i] = Gc[i] = 0.f;
+    unsigned fr0 = kSolid, fr1 = kSolid, fr2 = kSolid, fr3 = kSolid, fr4 = kSolid;
+    float pxa = 0.f, pya = 0.f, pxb = 0.f, pyb = 0.f, pxc = 0.f, pyc = 0.f;
+    // stage B: psi_n+2 of rows behind (qs) and current (qc), g of the current row
+    PsiRow qs{}, qc{};
+    float gq[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gq[i] = 0.f;
+    float rho_prev = 0.f;
+    // B -> C: g of B's rows r-2d (Ha), r-3d (Hb), r-4d (Hc) and their P
+    float Ha[9], Hb[9], Hc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ha[i] = Hb[i] = Hc[i] = 0.f;
+    float qxa = 0.f, qya = 0.f, qxb = 0.f, qyb = 0.f, qxc = 0.f, qyc = 0.f;
+    // stage C: psi_n+3 of rows behind (us) and current (uc), g of the current row
+    PsiRow us{}, uc{};
+    float gu[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gu[i] = 0.f;
+    float rho_prev_c = 0.f;
+
+    const int nj = (y1 - y0) + 8;
+    for (int j = 0; j < nj; ++j) {
+        const int r = ys + j * d;    // stage A row
+        const bool arow = !(k.y_walls && (r < 0 || r >= k.H));
+        const int rw = wrap(r, k.H);
+        // ---------------- stage A, row r
+        float Gn[9], pxn = 0.f, pyn = 0.f;
+        {
+            const int ya = r + d;
+            const unsigned fl2 = flags_of(k, b, ln, ya + d);
+            float ga[9], f[9];
+            const bool ga_on = g_on(k, ln, ya, fla);
+            pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
+            const bool f_on = arow && ln.inside && !((flc & kSolid) && k.inert);
+            pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, f_on, f);
+            const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, ga_on, ga));
+            const PsiRow pn = d > 0 ? pa : pb;  // row r + 1
+            const PsiRow ps = d > 0 ? pb : pa;  // row r - 1
+            const float nb9[9] = {pc.c, pc.r, pn.c, pc.l, ps.c, pn.r, pn.l, ps.l, ps.r};
+            float dpx, dpy, lap;
+            fdn::stencil(nb9, dpx, dpy, lap);
+            if (arow) bc_f(k, b, src, rw, (int)ln.x, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Gn[i] = gc[i];
+            const bool solid = !arow || !ln.inside || (flc & kSolid) != 0;
+            const float mu = collide_node(k, f, Gn, pc.c, dpx, dpy, lap, solid);
+            float* const slot = fring + (r & 3) * kRingSlot + lane;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) slot[i * 64] = f[i];
+            pxn = dpx * mu;
+            pyn = dpy * mu;
+            pb = pc;
+            pc = pa;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) gc[i] = ga[i];
+            fr4 = fr3;
+            fr3 = fr2;
+            fr2 = fr1;
+            fr1 = fr0;
+            fr0 = arow && ln.inside ? flc : kSolid;
+            flc = fla;
+            fla = fl2;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Gc[i] = Gb[i];
+            Gb[i] = Ga[i];
+            Ga[i] = Gn[i];
+        }
+        pxc = pxb;
+        pyc = pyb;
+        pxb = pxa;
+        pyb = pya;
+        pxa = pxn;
+        pya = pyn;
+        if (j < 2) continue;
+        // ---------------- stage B: psi_n+2 and g of row r-d
+        {
+            PsiRow qn;
+            float gn[9];
+            {
+                const int yq = r - d;
+                const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr1 & kSolid) && k.inert);
+                float lo[9], hi[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    lo[i] = d > 0 ? Gc[i] : Ga[i];
+                    hi[i] = d > 0 ? Ga[i] : Gc[i];
+                }
+                reg_pull(lo, Gb, hi, fr1 & kLinks, gn);
+                qn = lane_row(finish_g(k, ln, yq, fr1, on, gn));
+            }
+            if (j == 3 && has_in && k.y_walls && yfb - d >= 0 && yfb - d < k.H)
+                rho_prev = ring_inlet_rho(fring, yfb - d, lane, fr2 & kLinks, b.uprof[yfb - d], pxc);
+            float Hn[9], qxn = 0.f, qyn = 0.f;
+            if (j >= 4) {
+                // -------- stage B: collision of row y = r-2d into the rings
+                const int y = r - 2 * d;
+                const bool brow = !(k.y_walls && (y < 0 || y >= k.H));
+                const int yw = wrap(y, k.H);
+                const unsigned fl = fr2;
+                const bool solid = (fl & kSolid) != 0;
+                float f[9];
+                lds_pull(fring, y, lane, fl & kLinks, f);
+                float rho_nb = rho_prev;
+                if (has_in && brow) {
+                    if (k.y_walls && y == (d > 0 ? 0 : k.H - 1))
+                        rho_nb = ring_inlet_rho(fring, y + d, lane, fr1 & kLinks, b.uprof[y + d], pxb);
+                    rho_prev = fdn::inlet_rho(f, b.uprof[yw], pxc);
+                }
+                const PsiRow qa = d > 0 ? qn : qs;
+                const PsiRow qb = d > 0 ? qs : qn;
+                const float nb9[9] = {qc.c, qc.r, qa.c, qc.l, qb.c, qa.r, qa.l, qb.l, qb.r};
+                float dpx, dpy, lap;
+                fdn::stencil(nb9, dpx, dpy, lap);
+                if (brow) reg_bc_f(k, b, yw, (int)ln.x, f, pxc, pyc, rho_nb);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Hn[i] = gq[i];
+                const float mu = collide_node(k, f, Hn, qc.c, dpx, dpy, lap, solid || !ln.inside);
+                float* const slot = fring2 + (y & 3) * kRingSlot + lane;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) slot[i * 64] = f[i];
+                qxn = dpx * mu;
+                qyn = dpy * mu;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Hn[i] = 0.f;
+            }
+            qs = qc;
+            qc = qn;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                gq[i] = gn[i];
+                Hc[i] = Hb[i];
+                Hb[i] = Ha[i];
+                Ha[i] = Hn[i];
+            }
+            qxc = qxb;
+            qyc = qyb;
+            qxb = qxa;
+            qyb = qya;
+            qxa = qxn;
+            qya = qyn;
+        }
+        if (j < 6) continue;
+        // ---------------- stage C: psi_n+3 and g of row r-3d
+        PsiRow un;
+        float gv[9];
+        {
+            const int yq = r - 3 * d;
+            const bool on = ln.inside && !(k.y_walls && (yq < 0 || yq >= k.H)) && !((fr3 & kSolid) && k.inert);
+            float lo[9], hi[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                lo[i] = d > 0 ? Hc[i] : Ha[i];
+                hi[i] = d > 0 ? Ha[i] : Hc[i];
+            }
+            reg_pull(lo, Hb, hi, fr3 & kLinks, gv);
+            un = lane_row(finish_g(k, ln, yq, fr3, on, gv));
+        }
+        if (j == 6 && has_in && k.y_walls && yfc - d >= 0 && yfc - d < k.H)
+            rho_prev_c = ring_inlet_rho(fring2, yfc - d, lane, fr3 & kLinks, b.uprof[yfc - d], qxb);
+        if (j >= 8) {
+            // ------------ stage C: collision of row y = r-4d, stored
+            const int y = r - 4 * d;
+            const unsigned fl = fr4;
+            const bool solid = (fl & kSolid) != 0;
+            float f[9];
+            lds_pull(fring2, y, lane, fl & kLinks, f);
+            float rho_nb = rho_prev_c;
+            if (has_in) {
+                if (k.y_walls && y == (d > 0 ? 0 : k.H - 1))
+                    rho_nb = ring_inlet_rho(fring2, y + d, lane, fr3 & kLinks, b.uprof[y + d], qxb);
+                rho_prev_c = fdn::inlet_rho(f, b.uprof[y], qxc);
+            }
+            const PsiRow ua = d > 0 ? un : us;
+            const PsiRow ub = d > 0 ? us : un;
+            const float nb9[9] = {uc.c, uc.r, ua.c, uc.l, ub.c, ua.r, ua.l, ub.l, ub.r};
+            float dpx, dpy, lap;
+            fdn::stencil(nb9, dpx, dpy, lap);
+            reg_bc_f(k, b, y, (int)ln.x, f, qxc, qyc, rho_nb);
+            float g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] = gu[i];
+            const float mu = collide_node(k, f, g, uc.c, dpx, dpy, lap, solid || !ln.inside);
+            if (cout && zx && (ln.x == 0 || ln.x == (unsigned)k.W - 1)) {
+                const int sd = (ln.x == 0) ? 0 : 2;
+                b.side_out[sd * k.H + y] = dpx * mu;
+                b.side_out[(sd + 1) * k.H + y] = dpy * mu;
+            }
+            const unsigned vo = (cout && !(fl & kDeadPiece)) ? ln.x * 4u : kOOB;
+            const int rc = y * k.Wp;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, f[i]), dst, vo, (rc + i * k.Q) * 4,
+                                                      0);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, g[i]), dst, vo,
+                                                      (rc + (9 + i) * k.Q) * 4, 0);
+            }
+        }
+        us = uc;
+        uc = un;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gu[i] = gv[i];
+    }
+}
+
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void fused3_kernel(KParams k, KBufs b) {
+    fused3_march<WPB>(k, b, banded_block());
 }
 
 // One domain of a batched launch (an ensemble): its constants, flags,
@@ -1205,12 +1498,14 @@ struct fd_handle {
     bool init_u = false;  // started by fd_init_fields*: the velocity is staged in fields planes 2, 3
     int wx = 4;     // waves (64-column strips) per workgroup
     int rows = 16;  // rows per wave
-    int fuse = 2;   // iterations per launch of fd_step (1, or 2: fused_kernel)
+    int fuse = 2;   // iterations per launch of fd_step (1, 2: fused_kernel, 3: fused3_kernel)
     bool use_graph = false;            // fd_step replays captured launch pairs
     hipGraphExec_t gexec = nullptr;    // the captured pair
     int gkey[5] = {0, 0, 0, 0, 0};     // its launch configuration
     int fwx = 1;    // fused_kernel: waves per workgroup
     int frows = 12; // fused_kernel: output rows per wave
+    int f3wx = 1;   // fused3_kernel: waves per workgroup
+    int f3rows = 20; // fused3_kernel: output rows per wave
     int timed_launches = 0;
     bool have_timing = false;
     int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
@@ -1403,7 +1698,7 @@ bool geom_fusable(const fd_params& p) {
     return p.H >= 8 && edge_walls;
 }
 
-bool fusable(const fd_handle* h) { return h->fuse == 2 && geom_fusable(h->p); }
+bool fusable(const fd_handle* h) { return h->fuse >= 2 && geom_fusable(h->p); }
 
 // Workgroups across x and chunks of the two-iteration kernel on a domain of
 // width W and height H with launch shape wx x rows.
@@ -1428,6 +1723,68 @@ int launch_fused(fd_handle* h, const KBufs& b, hipStream_t stream = nullptr, int
     return FD_OK;
 }
 
+// Waves of the three-iteration kernel the device holds at once (one-wave
+// workgroups): the quantum of its launch rounds.
+int fused3_slots(int dev, const hipDeviceProp_t& prop, int* slots) {
+    int per_cu = 0;
+    FD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&fused3_kernel<1>), 64,
+                                                        0));
+    (void)dev;
+    *slots = std::max(1, per_cu) * std::max(1, prop.multiProcessorCount);
+    return FD_OK;
+}
+
+// Rows per wave of the three-iteration kernel on an H x W domain with
+// `slots` concurrent waves.  Its waves are bound by their own arithmetic
+// and run in rounds of `slots`: a launch costs about rounds x (rows + 8)
+// stage-A rows (the 8 rows beyond the chunk that stages A and B add), with
+// launches of fewer than three rounds 15 % dearer (their waves do not
+// finish together); the chosen height is the one with the least such cost
+// over 8..48 rows (the smallest on a tie).  Measured on 4096 x 2048 (76
+// strips, 2048 slots), ms per iteration: 26 rows (6,004 waves, 2.93
+// rounds) 0.164, 39 rows (1.97 rounds) 0.167, 20 rows (3.82) 0.170, 25
+// rows (3.04) 0.175, 52 rows (1.48) 0.187.
+int fused3_rows(int W, int H, int slots) {
+    const int64_t strips = (W + kFuse3Cols - 1) / kFuse3Cols;
+    int best = 8;
+    double best_cost = -1.0;
+    for (int rows = 8; rows <= 48; ++rows) {
+        const int64_t waves = strips * ((H + rows - 1) / rows);
+        const int64_t rounds = (waves + slots - 1) / slots;
+        const double cost = (double)(rounds * (rows + 8)) * (rounds < 3 ? 1.15 : 1.0);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = rows;
+        }
+    }
+    return best;
+}
+
+// One launch of the three-iteration kernel (one or two waves per
+// workgroup).
+int launch_fused3(fd_handle* h, const KBufs& b, hipStream_t stream = nullptr) {
+    const int wx = h->f3wx >= 2 ? 2 : 1;
+    KParams k = h->k;
+    k.rows = h->f3rows;
+    const int nstrips = (h->p.W + kFuse3Cols - 1) / kFuse3Cols;
+    k.nsb = (nstrips + wx - 1) / wx;
+    k.sb0 = 0;
+    const int nchunks = (h->p.H + h->f3rows - 1) / h->f3rows;
+    const dim3 grid(k.nsb * nchunks), block(64 * wx);
+    if (!stream) stream = h->stream;
+    if (wx == 2)
+        hipLaunchKernelGGL((fused3_kernel<2>), grid, block, 0, stream, k, b);
+    else
+        hipLaunchKernelGGL((fused3_kernel<1>), grid, block, 0, stream, k, b);
+    FD_HIP(hipGetLastError());
+    return FD_OK;
+}
+
+// One launch of `it` iterations (1, 2 or 3) from the buffers b.
+int launch_iterations(fd_handle* h, int it, const KBufs& b) {
+    return it == 3 ? launch_fused3(h, b) : (it == 2 ? launch_fused(h, b) : launch_step(h, 0, b));
+}
+
 // Buffers of one launch from population parity c and side parity sc.
 KBufs step_bufs(const fd_handle* h, int c, int sc) {
     KBufs b{};
@@ -1443,7 +1800,7 @@ KBufs step_bufs(const fd_handle* h, int c, int sc) {
 // The captured pair of launches of `per` iterations each, from parity 0:
 // (re)captured when the launch configuration changed since the last one.
 int step_graph(fd_handle* h, int per) {
-    const int key[5] = {per, h->fwx, h->frows, h->wx, h->rows};
+    const int key[5] = {per, per == 3 ? h->f3wx : h->fwx, per == 3 ? h->f3rows : h->frows, h->wx, h->rows};
     if (h->gexec && std::memcmp(key, h->gkey, sizeof key) == 0) return FD_OK;
     if (h->gexec) {
         (void)hipGraphExecDestroy(h->gexec);
@@ -1452,8 +1809,7 @@ int step_graph(fd_handle* h, int per) {
     hipGraph_t g = nullptr;
     FD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = FD_OK;
-    for (int p = 0; p < 2 && !rc; ++p)
-        rc = per == 2 ? launch_fused(h, step_bufs(h, p, p)) : launch_step(h, 0, step_bufs(h, p, p));
+    for (int p = 0; p < 2 && !rc; ++p) rc = launch_iterations(h, per, step_bufs(h, p, p));
     const hipError_t ec = hipStreamEndCapture(h->stream, &g);
     if (rc) {
         if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -1631,7 +1987,7 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
     fill_kparams(h);
     // Launch shapes: rows per wave chosen so that a launch has enough waves
     // to fill the device (about 8192 for one iteration per launch, 12288 for
-    // two; measured on MI355X: 4 x 16 and 1 x 12 on 4096 x 2048, 1 row per
+    // two; measured on MI355X: 4 x 16 and 1 x 8 on 4096 x 2048, 1 row per
     // wave on 400 x 100).  fd_set_tile / fd_set_fusion override them.
     {
         const int64_t strips64 = (p->W + 63) / 64, strips58 = (p->W + kFuseCols - 1) / kFuseCols;
@@ -1639,6 +1995,18 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
         h->rows = (int)std::min<int64_t>(16, std::max<int64_t>(1, (strips64 * p->H + 4096) / 8192));
         h->fwx = 1;
         h->frows = (int)std::min<int64_t>(8, std::max<int64_t>(1, (strips58 * p->H + 6144) / 12288));
+        int slots = 0;
+        if ((rc = fused3_slots(dev, prop, &slots))) {
+            delete h;
+            return rc;
+        }
+        h->f3rows = fused3_rows(p->W, p->H, slots);
+        // three iterations per launch on domains that fill the device at
+        // least twice over with waves of >= 16 rows (measured on 4096 x 2048:
+        // 0.164 ms per iteration at 1 x 26 against 0.175 with two)
+        if (geom_fusable(*p) && h->f3rows >= 16 &&
+            (int64_t)((p->W + kFuse3Cols - 1) / kFuse3Cols) * ((p->H + h->f3rows - 1) / h->f3rows) >= 2 * (int64_t)slots)
+            h->fuse = 3;
     }
 
     const size_t Q = h->Q, H = p->H;
@@ -1767,7 +2135,7 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         return rc;
     };
     auto issue = [&](int it) {
-        int rc = it == 2 ? launch_fused(h, step_bufs(h, h->cur, h->sc)) : launch_step(h, 0, step_bufs(h, h->cur, h->sc));
+        int rc = launch_iterations(h, it, step_bufs(h, h->cur, h->sc));
         if (rc) return rc;
         h->cur ^= 1;
         h->sc ^= 1;
@@ -1775,9 +2143,9 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         return FD_OK;
     };
     if (nl > 0) FD_HIP(hipEventRecord(h->ev0, h->stream));
-    // pairs of iterations go to the two-iteration kernel; long loops replay
-    // a captured pair of launches (buffer parity 0 -> 1 -> 0)
-    const int per = fusable(h) ? 2 : 1;
+    // groups of iterations go to the two- or three-iteration kernel; long
+    // loops replay a captured pair of launches (buffer parity 0 -> 1 -> 0)
+    const int per = fusable(h) ? h->fuse : 1;
     int rc = FD_OK;
     if (h->use_graph && h->cur == h->sc && nl >= 4 * per) {
         if (h->cur != 0 && (rc = issue(per))) return failed(rc);
@@ -1788,7 +2156,7 @@ int fd_step(fd_handle* h, int32_t nsteps) {
         }
     }
     while (i < nl)
-        if ((rc = issue(per == 2 && nl - i >= 2 ? 2 : 1))) return failed(rc);
+        if ((rc = issue((int)std::min<int64_t>(per, nl - i)))) return failed(rc);
     if (nl > 0) {
         FD_HIP(hipEventRecord(h->ev1, h->stream));
         h->timed_launches = (int)nl;
@@ -1873,12 +2241,26 @@ double fd_last_step_ms(const fd_handle* h) {
 
 int fd_set_fusion(fd_handle* h, int32_t depth, int32_t waves_x, int32_t rows) {
     if (!h) return fail(FD_E_ARG, "handle is NULL");
-    if (depth != 1 && depth != 2) return fail(FD_E_ARG, "depth must be 1 or 2");
+    if (depth < 1 || depth > 3) return fail(FD_E_ARG, "depth must be 1, 2 or 3");
     if (waves_x > 0 && waves_x != 1 && waves_x != 2 && waves_x != 4)
         return fail(FD_E_ARG, "waves_x must be 1, 2 or 4 (or <= 0 to keep)");
     h->fuse = depth;
-    if (waves_x > 0) h->fwx = waves_x;
-    if (rows > 0) h->frows = rows;
+    if (depth == 3) {
+        if (waves_x > 0) h->f3wx = waves_x;
+        if (rows > 0) h->f3rows = rows;
+    } else {
+        if (waves_x > 0) h->fwx = waves_x;
+        if (rows > 0) h->frows = rows;
+    }
+    return FD_OK;
+}
+
+int fd_get_fusion(const fd_handle* h, int32_t* depth, int32_t* waves_x, int32_t* rows) {
+    if (!h) return fail(FD_E_ARG, "handle is NULL");
+    const int d = fusable(h) ? h->fuse : 1;
+    if (depth) *depth = d;
+    if (waves_x) *waves_x = d == 3 ? h->f3wx : (d == 2 ? h->fwx : h->wx);
+    if (rows) *rows = d == 3 ? h->f3rows : (d == 2 ? h->frows : h->rows);
     return FD_OK;
 }
 

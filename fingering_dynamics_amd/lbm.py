@@ -128,13 +128,23 @@ class Compute:
         _native.check(self._lib.fd_use_graph(self._h, 1 if on else 0))
 
     def set_fusion(self, depth, waves_x=0, rows=0):
-        """Iterations per launch: 2 (default) keeps the intermediate state of
-        each pair of iterations in registers; 1 runs one launch per
-        iteration.  Identical results.  waves_x / rows (> 0) set the
-        two-iteration kernel's launch shape."""
+        """Iterations per launch: 3 or 2 keep the intermediate states of
+        each group of iterations on chip; 1 runs one launch per iteration.
+        Identical results.  waves_x / rows (> 0) set that depth's launch
+        shape.  The default depends on the domain size (``fusion``)."""
         _native.check(self._lib.fd_set_fusion(self._h, int(depth), int(waves_x), int(rows)))
 
-    TUNE_CANDIDATES = tuple((2, 1, r) for r in (1, 2, 4, 8, 12, 16, 24)) + ((2, 4, 16),) + \
+    @property
+    def fusion(self):
+        """(iterations per launch, waves per workgroup, rows per wave) that
+        step() uses now."""
+        d, w, r = (_native.ctypes.c_int32() for _ in range(3))
+        _native.check(self._lib.fd_get_fusion(self._h, _native.ctypes.byref(d), _native.ctypes.byref(w),
+                                              _native.ctypes.byref(r)))
+        return d.value, w.value, r.value
+
+    TUNE_CANDIDATES = tuple((3, 1, r) for r in (16, 20, 26, 39)) + \
+        tuple((2, 1, r) for r in (1, 2, 4, 8, 12, 16, 24)) + ((2, 4, 16),) + \
         tuple((1, wx, r) for wx, r in ((1, 1), (4, 1), (4, 4), (4, 16)))
 
     def autotune(self, candidates=TUNE_CANDIDATES, steps=20):
@@ -143,13 +153,15 @@ class Compute:
         on it, so the trial iterations are part of the run.  Returns
         {(depth, waves, rows): ms per iteration}."""
         times = {}
+        steps -= steps % 6
+        steps = max(steps, 6)
         for depth, wx, rows in candidates:
             if depth == 1:
                 self.set_fusion(1)
                 self.set_tile(wx, rows)
             else:
-                self.set_fusion(2, wx, rows)
-            self.step(2)
+                self.set_fusion(depth, wx, rows)
+            self.step(6)
             self.step(steps)
             times[(depth, wx, rows)] = self.last_step_ms()
         best = min(times, key=times.get)
@@ -157,7 +169,7 @@ class Compute:
             self.set_fusion(1)
             self.set_tile(*best[1:])
         else:
-            self.set_fusion(2, *best[1:])
+            self.set_fusion(*best)
         return times
 
     # ------------------------------------------------------------ outputs

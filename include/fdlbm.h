@@ -231,17 +231,25 @@ double fd_last_step_ms(const fd_handle* h);
  * implementation). */
 int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
 
-/* Iterations per kernel launch of fd_step: depth 2 (the default) advances
- * pairs of iterations with one launch that keeps the intermediate state in
- * registers, so each iteration reads and writes HBM once per two
- * iterations; depth 1 runs one launch per iteration.  The results are
- * identical bit for bit.  waves_x (1, 2 or 4) and rows set the two-iteration
- * kernel's launch shape (<= 0 keeps the current one; default 1 wave x 8 rows,
- * fewer rows on domains too small to fill the device).
- * Domains with H < 8, and walled domains whose walls are not the edge rows
- * (the rows 1 and H-2 of fingering.py:573), always run depth 1.  No reference counterpart (a
+/* Iterations per kernel launch of fd_step: depth 2 advances pairs of
+ * iterations with one launch that keeps the intermediate state on chip, so
+ * the populations cross HBM once per two iterations; depth 3 does the same
+ * for triples (two intermediate states on chip); depth 1 runs one launch
+ * per iteration.  The results are identical bit for bit.  waves_x (1, 2 or
+ * 4; depth 3 uses 1 or 2) and rows set the launch shape of that depth's
+ * kernel (<= 0 keeps the current one).  Defaults: depth 3 at a height
+ * chosen for whole launch rounds on domains that fill the device at least
+ * twice over (26 rows on 4096 x 2048), else depth 2 at 1 wave x 8 rows,
+ * fewer rows on domains too small to fill the device.  Domains with H < 8,
+ * and walled domains whose walls are not the edge rows (the rows 1 and H-2
+ * of fingering.py:573), always run depth 1.  No reference counterpart (a
  * launch control of this implementation). */
 int fd_set_fusion(fd_handle* h, int32_t depth, int32_t waves_x, int32_t rows);
+
+/* The iterations per launch fd_step uses now (1 when the domain does not
+ * allow more) and that kernel's launch shape; NULL outputs are skipped.
+ * No reference counterpart. */
+int fd_get_fusion(const fd_handle* h, int32_t* depth, int32_t* waves_x, int32_t* rows);
 
 /* 1 if the handle runs with inert solid nodes, 0 if solid nodes stream
  * their populations on (store-and-forward, fingering_periodic.py:455-466),

@@ -1,13 +1,15 @@
-"""GPU: the two-iteration launch (fd_set_fusion depth 2, the default of
-fd_step) equals one launch per iteration (depth 1) bit for bit.
+"""GPU: the two- and three-iteration launches (fd_set_fusion depth 2, the
+default of fd_step, and depth 3) equal one launch per iteration (depth 1)
+bit for bit.
 
-Compared after odd and even step counts: the checkpoint blob (post-collision
-populations of every node and the Zou-He side values), the fields and the
-post-stream populations.  Covered: every boundary family of the parity
-cases, inert and store-and-forward solids, domain widths below, at and
-across the 58-column strip of the two-iteration kernel, periodic x with
-several wraps per strip, chunk heights from 1 row to more than the domain,
-and every launch shape.
+Compared after step counts that leave every remainder: the checkpoint blob
+(post-collision populations of every node and the Zou-He side values), the
+fields and the post-stream populations.  Covered: every boundary family of
+the parity cases, inert and store-and-forward solids, domain widths below,
+at and across the 58- and 54-column strips of the two kernels, periodic x
+with several wraps per strip, chunk heights from 1 row to more than the
+domain (stage rows beyond walled edges and wrapping periodic ones), and
+every launch shape.
 """
 import numpy as np
 import pytest
@@ -67,59 +69,65 @@ def _run(one, two, counts, tag):
     two.close()
 
 
+@pytest.mark.parametrize("depth", [2, 3])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_fused_equals_single_on_the_parity_cases(case):
+def test_fused_equals_single_on_the_parity_cases(case, depth):
     params, mask, bits, _ = _setup(case)
-    one, two = _pair(params, mask, bits)
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
     _run(one, two, (1, 2, 3, 6, 20), case)
 
 
-@pytest.mark.parametrize("W", [8, 37, 57, 58, 59, 61, 64, 115, 116, 117, 121, 131, 250])
+@pytest.mark.parametrize("depth", [2, 3])
+@pytest.mark.parametrize("W", [8, 37, 53, 54, 55, 57, 58, 59, 61, 64, 107, 108, 109, 115, 116, 117, 121, 131, 250])
 @pytest.mark.parametrize("periodic_x", [False, True])
-def test_fused_equals_single_across_widths(W, periodic_x):
+def test_fused_equals_single_across_widths(W, periodic_x, depth):
     H = 20
     mask, bits, _ = rect_domain(H, W, max(1, W // 30), size=3, seed=W, x_min=min(5, W // 3))
     kw = dict(M=10.0, psi_wall=-0.2, n_inv_cols=min(6, W // 2))
     if periodic_x:
         kw.update(x_bc=X_PERIODIC, gx=1e-5)
     params = LbmParams.periodic(H, W, **kw)
-    one, two = _pair(params, mask, bits)
-    _run(one, two, (2, 5, 8), "W=%d periodic_x=%s" % (W, periodic_x))
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
+    _run(one, two, (2, 5, 8), "W=%d periodic_x=%s depth=%d" % (W, periodic_x, depth))
 
 
+@pytest.mark.parametrize("depth", [2, 3])
 @pytest.mark.parametrize("shape", [(1, 1), (1, 3), (2, 7), (4, 16), (2, 48), (1, 64), (4, 200)])
-def test_fused_equals_single_across_launch_shapes(shape):
+def test_fused_equals_single_across_launch_shapes(shape, depth):
     H, W = 40, 150
     mask, bits, _ = rect_domain(H, W, 10, size=4, seed=11, x_min=15)
     params = LbmParams.walled(H, W, shipped_walls=False, M=10.0, psi_wall=-0.5, gamma=1.0027,
                               outlet=OUTLET_PRESSURE, outlet_f3_coef=2.0 / 3.0)
-    one, two = _pair(params, mask, bits, fused=(2,) + shape)
-    _run(one, two, (4, 9), "shape=%s" % (shape,))
+    one, two = _pair(params, mask, bits, fused=(depth,) + shape)
+    _run(one, two, (4, 9), "shape=%s depth=%d" % (shape, depth))
 
 
+@pytest.mark.parametrize("depth", [2, 3])
 @pytest.mark.parametrize("H", [8, 9, 13, 50])
-def test_fused_equals_single_short_domains(H):
+def test_fused_equals_single_short_domains(H, depth):
     W = 70
     mask, bits, links = rect_domain(H, W, 2, size=2, seed=H, x_min=12)
     params = LbmParams.walled(H, W, shipped_walls=False, M=20.0, psi_wall=-1.0, gamma=1.0027)
-    one, two = _pair(params, mask, bits)
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
     _run(one, two, (1, 4, 7), "H=%d" % H)
     # periodic y at the same heights
     params = LbmParams.periodic(H, W, M=5.0, psi_wall=0.3, n_inv_cols=10, profile=PROFILE_CONST)
-    one, two = _pair(params, mask, bits)
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
     _run(one, two, (2, 5), "H=%d periodic" % H)
 
 
-def test_fused_equals_single_on_the_bench_workload_copy():
+@pytest.mark.parametrize("depth", [2, 3])
+def test_fused_equals_single_on_the_bench_workload_copy(depth):
     mask, bits, params, _ = bench.workload(256, 512)
-    one, two = _pair(params, mask, bits)
-    _run(one, two, (2, 20), "bench 512x256")
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
+    _run(one, two, (2, 20, 3), "bench 512x256")
 
 
-def test_fused_equals_single_after_state_hand_over():
+@pytest.mark.parametrize("depth", [2, 3])
+def test_fused_equals_single_after_state_hand_over(depth):
     """The first fused launch after fd_set_populations / fd_init_fields."""
     params, mask, bits, _ = _setup("walled_pressure")
-    one, two = _pair(params, mask, bits)
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
     one.step(5)
     two.step(5)
     f, g = one.populations()
@@ -138,7 +146,8 @@ def test_graph_replay_equals_eager_launches():
     eager.use_graph(False)
     graph = _compute(mask, bits, params)
     for n, change in ((9, None), (16, None), (5, ("fusion", (1, 0, 0))), (12, None), (7, ("fusion", (2, 1, 5))),
-                      (20, ("tile", (2, 3))), (8, None), (1, None), (13, None)):
+                      (20, ("tile", (2, 3))), (8, None), (1, None), (13, None), (14, ("fusion", (3, 1, 4))),
+                      (25, None), (2, None), (13, ("fusion", (3, 2, 7)))):
         if change:
             for cm in (eager, graph):
                 getattr(cm, "set_" + change[0])(*change[1])
