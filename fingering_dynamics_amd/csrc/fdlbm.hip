@@ -915,7 +915,7 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
 // Each stage needs two columns more of the stage before it, so the 64
 // lanes hold columns x0 - 5 ... x0 + 58 and stage C's 54 output columns
 // are lanes 5..58.  For the chunk [y0, y1) stage A runs rows y0-4 ... y1+3
-// and stage B rows y0-2 ... y1+1, which may lie beyond a walled domain's
+// (upwards) and stage B rows y0-2 ... y1+1, which may lie beyond a walled domain's
 // edge rows (ghost rows: psi_wall, no populations) or wrap around a
 // periodic one.  The arithmetic of every node is the single-iteration
 // kernel's, in the same order, so the result equals three single-iteration
@@ -946,7 +946,13 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
     float* const fring = rings_all + wave * 2 * kRingRegion + 1;    // A -> B
     float* const fring2 = fring + kRingRegion;                       // B -> C
 
-    const int d = (chunk & 1) ? -1 : 1;
+    // Every chunk marches up.  This kernel is bound by its arithmetic, not
+    // by HBM, so the L2 reuse that alternating directions buy the
+    // two-iteration kernel is worth less here than the direction selects
+    // it costs (measured on 4096 x 2048 at 1 x 26: 0.1617 ms per iteration
+    // up-only, 0.1641 alternating).  The row expressions keep d for the
+    // reading.
+    constexpr int d = 1;
     const int ys = d > 0 ? y0 - 4 : y1 + 3;   // stage A's first row
     const int yfb = ys + 2 * d;               // stage B's first row
     const int yfc = ys + 4 * d;               // stage C's first row
@@ -1004,11 +1010,16 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         {
             const int ya = r + d;
             const unsigned fl2 = flags_of(k, b, ln, ya + d);
+            // every lane loads: what inert solids, ghost rows and lanes
+            // beyond the domain load is never used by a node of the domain
+            // (psi and the collision of those lanes do not depend on it, and
+            // every pull of a domain node from them is a bounce-back link or
+            // rewritten by Zou-He), and inert solid nodes hold zeros in
+            // memory, which they keep; so no load needs a mask here
             float ga[9], f[9];
             const bool ga_on = g_on(k, ln, ya, fla);
-            pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, ga_on, ga);
-            const bool f_on = arow && ln.inside && !((flc & kSolid) && k.inert);
-            pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, f_on, f);
+            pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, true, ga);
+            pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, true, f);
             const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, ga_on, ga));
             const PsiRow pn = d > 0 ? pa : pb;  // row r + 1
             const PsiRow ps = d > 0 ? pb : pa;  // row r - 1
@@ -1246,6 +1257,19 @@ __global__ __launch_bounds__(64 * WPB) void ens_fused_kernel(const EnsDomain* __
     b.side_out = e.side[ps ^ 1];
     b.uprof = e.uprof;
     fused_march<WPB>(e.k, b, L - e.blk0);
+}
+
+// Zeros on every solid node of an inert-solid handle's population set:
+// the state the step kernels keep on them (restored blobs are made to
+// agree).
+__global__ __launch_bounds__(256) void zero_inert_solids(float* __restrict__ pop, const uint16_t* __restrict__ flags,
+                                                         int H, int W, int Wp, int64_t Q) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int64_t o = (int64_t)y * Wp + x;
+    if (!(flags[o] & kSolid)) return;
+#pragma unroll
+    for (int i = 0; i < 18; ++i) pop[i * Q + o] = 0.f;
 }
 
 // Initial state (fingering_periodic.py:90-121, or fingering.py:95-127 with
@@ -2490,6 +2514,11 @@ int fd_load_state(fd_handle* h, const void* buf, int64_t nbytes) {
                                 sizeof(float) * h->p.W, h->p.H, hipMemcpyHostToDevice, h->stream));
         p += sizeof(float) * (size_t)h->p.H * h->p.W;
     }
+    if (h->inert) {
+        hipLaunchKernelGGL(zero_inert_solids, dim3((h->p.W + 63) / 64, (h->p.H + 3) / 4), dim3(64, 4), 0, h->stream,
+                           h->pop[h->cur], h->flags, h->p.H, h->p.W, h->Wp, (int64_t)h->Q);
+        FD_HIP(hipGetLastError());
+    }
     FD_HIP(hipStreamSynchronize(h->stream));
     h->N = hdr[3];
     h->inited = true;
@@ -2607,6 +2636,11 @@ int fd_load_state_device(fd_handle* h, const void* dbuf, int64_t nbytes, int64_t
     FD_HIP(hipMemcpyAsync(h->side[h->sc], p, sizeof(float) * 4 * h->p.H, hipMemcpyDeviceToDevice, h->stream));
     FD_HIP(hipMemcpyAsync(h->pop[h->cur], p + sizeof(float) * 4 * h->p.H, sizeof(float) * 18 * h->Q,
                           hipMemcpyDeviceToDevice, h->stream));
+    if (h->inert) {
+        hipLaunchKernelGGL(zero_inert_solids, dim3((h->p.W + 63) / 64, (h->p.H + 3) / 4), dim3(64, 4), 0, h->stream,
+                           h->pop[h->cur], h->flags, h->p.H, h->p.W, h->Wp, (int64_t)h->Q);
+        FD_HIP(hipGetLastError());
+    }
     h->N = steps_done;
     h->inited = true;
     h->stage = 0;

@@ -156,3 +156,17 @@ def test_graph_replay_equals_eager_launches():
         _same(eager, graph, "graph t=%d" % eager.t)
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("depth", [2, 3])
+def test_fused_after_checkpoint_restore(depth):
+    """A checkpoint restored into a fresh handle continues at depth 2 or 3
+    exactly as the handle it was taken from continues at depth 1 (inert
+    solids, walls, pressure outlet: the bench's boundary family)."""
+    mask, bits, params, _ = bench.workload(256, 512)
+    one, two = _pair(params, mask, bits, fused=(depth, 0, 0))
+    one.step(7)
+    blob = one.save_state()
+    two.load_state(blob)
+    assert two.t == 7
+    _run(one, two, (3, 8, 1), "restored depth=%d" % depth)
