@@ -152,7 +152,7 @@ def ensemble_bench(n, steps, warmup):
     ens.synchronize()
     nodes = ens.nodes
     res = {}
-    fusion, fused = ens.fusion, ens.fused
+    fusion, depth = ens.fusion, ens.depth
     for mode in ("graph", "eager", "sequential", "one_iteration_per_launch"):
         k = steps if mode != "sequential" else max(steps // 8, 10)
         k += k & 1
@@ -176,7 +176,7 @@ def ensemble_bench(n, steps, warmup):
         "batched_graph": res["graph"], "batched_eager": res["eager"], "sequential": res["sequential"],
         "batched_one_iteration_per_launch": res["one_iteration_per_launch"],
         "speedup_vs_sequential": round(res["graph"]["mlups"] / res["sequential"]["mlups"], 2),
-        "iterations_per_launch": 2 if fused else 1,
+        "iterations_per_launch": depth,
         "autotune_ms_per_step": {"%dx%d" % t: round(v, 4) for t, v in tune.items()},
         "shape": "%dx%d" % fusion[1:], "tile_one_iteration": "%dx%d" % ens.tile, "setup_s": round(setup_s, 1),
         "domains_non_finite": int(sum(1 for v in bad if v)), "iterations_run": int(cs[0].t),

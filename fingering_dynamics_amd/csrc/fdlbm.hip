@@ -1236,9 +1236,10 @@ __global__ __launch_bounds__(64 * WPB) void ens_kernel(const EnsDomain* __restri
     march_block<WPB, 0>(e.k, b, L - e.blk0);
 }
 
-// Two iterations of every domain of an ensemble in one launch: as
-// ens_kernel, over a table laid out for the two-iteration march.
-template <int WPB>
+// Two (DEPTH 2) or three (DEPTH 3) iterations of every domain of an
+// ensemble in one launch: as ens_kernel, over a table laid out for that
+// march.
+template <int WPB, int DEPTH>
 __global__ __launch_bounds__(64 * WPB) void ens_fused_kernel(const EnsDomain* __restrict__ doms, int ndom,
                                                              int parity) {
     const int L = banded_block();
@@ -1256,7 +1257,10 @@ __global__ __launch_bounds__(64 * WPB) void ens_fused_kernel(const EnsDomain* __
     b.side_in = e.side[ps];
     b.side_out = e.side[ps ^ 1];
     b.uprof = e.uprof;
-    fused_march<WPB>(e.k, b, L - e.blk0);
+    if (DEPTH == 3)
+        fused3_march<WPB>(e.k, b, L - e.blk0);
+    else
+        fused_march<WPB>(e.k, b, L - e.blk0);
 }
 
 // Zeros on every solid node of an inert-solid handle's population set:
@@ -2660,10 +2664,11 @@ struct fd_ensemble {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evd = nullptr;
     EnsDomain* table = nullptr;      // single-iteration launch
-    EnsDomain* ftable = nullptr;     // two-iteration launch
-    int nblocks = 0, fnblocks = 0;
+    EnsDomain* ftable = nullptr;     // two- or three-iteration launch (fdepth)
+    int nblocks = 0, fnblocks = 0, fdepth = 0;
     int wx = 4, rows = 16;           // single-iteration launch shape
-    int fuse = 2, fwx = 1, frows = 8;  // iterations per launch, two-iteration launch shape
+    int fuse = 3, fwx = 1, frows = 8;  // iterations per launch, two-iteration launch shape
+    int f3wx = 1, f3rows = 26;       // three-iteration launch shape
     bool table_ok = false;
     int64_t flips = 0;               // launches since the table was built
     bool use_graph = true;
@@ -2675,13 +2680,13 @@ struct fd_ensemble {
 
 namespace {
 
-// Pairs of iterations go to the two-iteration kernel when every domain's
-// geometry allows it.
-bool ens_fused(const fd_ensemble* e) {
-    if (e->fuse != 2) return false;
+// Iterations per batched launch: e->fuse when every domain's geometry
+// allows the two- and three-iteration kernels, else 1.
+int ens_depth(const fd_ensemble* e) {
+    if (e->fuse < 2) return 1;
     for (const fd_handle* h : e->hs)
-        if (!geom_fusable(h->p)) return false;
-    return true;
+        if (!geom_fusable(h->p)) return 1;
+    return e->fuse;
 }
 
 int ens_build(fd_ensemble* e) {
@@ -2692,7 +2697,8 @@ int ens_build(fd_ensemble* e) {
         }
     std::vector<EnsDomain> t(e->hs.size()), ft;
     int64_t blk = 0, fblk = 0;
-    const bool fused = ens_fused(e);
+    const int depth = ens_depth(e);
+    const bool fused = depth > 1;
     if (fused) ft.resize(e->hs.size());
     for (size_t i = 0; i < e->hs.size(); ++i) {
         fd_handle* h = e->hs[i];
@@ -2716,10 +2722,12 @@ int ens_build(fd_ensemble* e) {
         if (fused) {
             EnsDomain& fd = ft[i];
             fd = d;
-            fd.k.rows = e->frows;
-            fd.k.nsb = fused_strip_groups(h->p.W, e->fwx);
+            const int frows = depth == 3 ? e->f3rows : e->frows;
+            fd.k.rows = frows;
+            fd.k.nsb = depth == 3 ? ((h->p.W + kFuse3Cols - 1) / kFuse3Cols + e->f3wx - 1) / e->f3wx
+                                  : fused_strip_groups(h->p.W, e->fwx);
             fd.blk0 = (int)fblk;
-            fblk += (int64_t)fd.k.nsb * ((h->p.H + e->frows - 1) / e->frows);
+            fblk += (int64_t)fd.k.nsb * ((h->p.H + frows - 1) / frows);
         }
     }
     if (blk >= ((int64_t)1 << 31) || fblk >= ((int64_t)1 << 31))
@@ -2731,19 +2739,30 @@ int ens_build(fd_ensemble* e) {
     FD_HIP(hipStreamSynchronize(e->stream));
     e->nblocks = (int)blk;
     e->fnblocks = fused ? (int)fblk : 0;
+    e->fdepth = depth;
     e->flips = 0;
     e->table_ok = true;
     return FD_OK;
 }
 
-int ens_launch(fd_ensemble* e, int parity, bool fused = false) {
+// One batched launch of `it` iterations (1, or the table's depth).
+int ens_launch(fd_ensemble* e, int parity, int it = 1) {
     const int n = (int)e->hs.size();
-    if (fused) {
+    if (it == 3) {
+        const dim3 grid(e->fnblocks), block(64 * e->f3wx);
+        if (e->f3wx == 2)
+            hipLaunchKernelGGL((ens_fused_kernel<2, 3>), grid, block, 0, e->stream, e->ftable, n, parity);
+        else
+            hipLaunchKernelGGL((ens_fused_kernel<1, 3>), grid, block, 0, e->stream, e->ftable, n, parity);
+        FD_HIP(hipGetLastError());
+        return FD_OK;
+    }
+    if (it == 2) {
         const dim3 grid(e->fnblocks), block(64 * e->fwx);
         switch (e->fwx) {
-            case 1: hipLaunchKernelGGL((ens_fused_kernel<1>), grid, block, 0, e->stream, e->ftable, n, parity); break;
-            case 4: hipLaunchKernelGGL((ens_fused_kernel<4>), grid, block, 0, e->stream, e->ftable, n, parity); break;
-            default: hipLaunchKernelGGL((ens_fused_kernel<2>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+            case 1: hipLaunchKernelGGL((ens_fused_kernel<1, 2>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+            case 4: hipLaunchKernelGGL((ens_fused_kernel<4, 2>), grid, block, 0, e->stream, e->ftable, n, parity); break;
+            default: hipLaunchKernelGGL((ens_fused_kernel<2, 2>), grid, block, 0, e->stream, e->ftable, n, parity); break;
         }
         FD_HIP(hipGetLastError());
         return FD_OK;
@@ -2761,13 +2780,13 @@ int ens_launch(fd_ensemble* e, int parity, bool fused = false) {
 
 // Two launches (parity 0, then 1) captured once and replayed: the buffer
 // sets alternate, so a pair returns every pointer to where it started.
-int ens_graph(fd_ensemble* e, bool fused) {
-    hipGraphExec_t& exec = fused ? e->fexec : e->exec;
+int ens_graph(fd_ensemble* e, int per) {
+    hipGraphExec_t& exec = per > 1 ? e->fexec : e->exec;
     if (exec) return FD_OK;
     hipGraph_t g = nullptr;
     FD_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ens_launch(e, 0, fused);
-    if (!rc) rc = ens_launch(e, 1, fused);
+    int rc = ens_launch(e, 0, per);
+    if (!rc) rc = ens_launch(e, 1, per);
     const hipError_t ec = hipStreamEndCapture(e->stream, &g);
     if (rc) {
         if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -2807,15 +2826,23 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
             FD_HIP(hipStreamWaitEvent(e->stream, h->evs, 0));
         }
         FD_HIP(hipEventRecord(e->ev0, e->stream));
-        // iterations per launch: pairs go to the two-iteration kernel
-        const int per = ens_fused(e) ? 2 : 1;
+        // iterations per launch: groups go to the two- or three-iteration
+        // kernel, a shorter remainder to the single-iteration one
+        const int per = e->fdepth > 1 ? e->fdepth : 1;
         if (sequential) {
             // the baseline: one launch per domain per launch step, one stream
             for (int64_t i = 0; i < nl;) {
-                const int it = (per == 2 && nl - i >= 2) ? 2 : 1;
+                const int it = nl - i >= per ? per : 1;
                 for (fd_handle* h : e->hs) {
                     const KBufs b = step_bufs(h, h->cur, h->sc);
-                    if (it == 2) {
+                    if (it == 3) {
+                        const int wx3 = h->f3wx, rows3 = h->f3rows;
+                        h->f3wx = e->f3wx;
+                        h->f3rows = e->f3rows;
+                        rc = launch_fused3(h, b, e->stream);
+                        h->f3wx = wx3;
+                        h->f3rows = rows3;
+                    } else if (it == 2) {
                         rc = launch_fused(h, b, e->stream, e->fwx, e->frows);
                     } else {
                         const int wx = h->wx, rows = h->rows;
@@ -2833,10 +2860,10 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
                 ++e->flips;
             }
         } else {
-            // every launch, of one iteration or two, flips the buffer parity
+            // every launch, of one iteration or more, flips the buffer parity
             int64_t i = 0, launches = 0;
             auto one = [&](int it) {
-                const int r = ens_launch(e, (int)(e->flips & 1), it == 2);
+                const int r = ens_launch(e, (int)(e->flips & 1), it);
                 if (!r) {
                     i += it;
                     ++e->flips;
@@ -2844,14 +2871,14 @@ int ens_step(fd_ensemble* e, int32_t nsteps, bool sequential) {
                 }
                 return r;
             };
-            if ((e->flips & 1) != 0 && (rc = one(per == 2 && nl >= 2 ? 2 : 1))) return rc;
+            if ((e->flips & 1) != 0 && (rc = one(nl >= per ? per : 1))) return rc;
             if (e->use_graph && nl - i >= 2 * per) {
-                if ((rc = ens_graph(e, per == 2))) return rc;
+                if ((rc = ens_graph(e, per))) return rc;
                 for (; nl - i >= 2 * per; i += 2 * per, e->flips += 2, launches += 2)
-                    FD_HIP(hipGraphLaunch(per == 2 ? e->fexec : e->exec, e->stream));
+                    FD_HIP(hipGraphLaunch(per > 1 ? e->fexec : e->exec, e->stream));
             }
             while (i < nl)
-                if ((rc = one(per == 2 && nl - i >= 2 ? 2 : 1))) return rc;
+                if ((rc = one(nl - i >= per ? per : 1))) return rc;
             for (fd_handle* h : e->hs) {
                 h->cur ^= (int)(launches & 1);
                 h->sc ^= (int)(launches & 1);
@@ -2931,11 +2958,17 @@ int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows) {
 
 int fd_ens_set_fusion(fd_ensemble* e, int32_t depth, int32_t waves_x, int32_t rows) {
     if (!e) return fail(FD_E_ARG, "ensemble is NULL");
-    if (depth != 1 && depth != 2) return fail(FD_E_ARG, "depth must be 1 or 2");
+    if (depth < 1 || depth > 3) return fail(FD_E_ARG, "depth must be 1, 2 or 3");
     if (waves_x > 0 && waves_x != 1 && waves_x != 2 && waves_x != 4) return fail(FD_E_ARG, "waves_x must be 1, 2 or 4");
+    if (depth == 3 && waves_x > 2) return fail(FD_E_ARG, "three iterations per launch take 1 or 2 waves per workgroup");
     e->fuse = depth;
-    if (waves_x > 0) e->fwx = waves_x;
-    if (rows > 0) e->frows = rows;
+    if (depth == 3) {
+        if (waves_x > 0) e->f3wx = waves_x;
+        if (rows > 0) e->f3rows = rows;
+    } else {
+        if (waves_x > 0) e->fwx = waves_x;
+        if (rows > 0) e->frows = rows;
+    }
     e->table_ok = false;
     return FD_OK;
 }

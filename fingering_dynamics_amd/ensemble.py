@@ -5,9 +5,9 @@ The reference runs one domain per process (fingering_periodic.py:454-479).
 Here any number of ``Compute`` solvers on one GPU - different sizes,
 viscosity ratios, wall values, obstacles and boundary conditions - are
 batched: one kernel launch walks a device table of the domains and advances
-all of them by two iterations (the two-iteration kernel of fd_step, when
-every domain's geometry allows it) or by one, and step loops are replayed
-from a captured HIP graph.  ``step_sequential`` runs the same iterations as
+all of them by three iterations (the three-iteration kernel of fd_step, the
+default; two with set_fusion(2)) when every domain's geometry allows it, or
+by one, and step loops are replayed from a captured HIP graph.  ``step_sequential`` runs the same iterations as
 one launch per domain on the same stream, the baseline the batched launch
 is measured against; both give bit-identical states.
 
@@ -38,6 +38,7 @@ from .params import LbmParams, PROFILE_CONST, OUTLET_PRESSURE, Y_WALLS
 
 TILE_CANDIDATES = ((4, 16), (4, 8), (4, 32), (8, 16), (2, 16), (1, 16))
 FUSED_CANDIDATES = ((1, 8), (1, 6), (1, 12), (1, 16), (2, 8), (4, 8))
+FUSED3_CANDIDATES = ((1, 16), (1, 20), (1, 26), (1, 32), (1, 12), (2, 16))
 
 
 class Ensemble:
@@ -53,7 +54,8 @@ class Ensemble:
         _native.check(self._lib.fd_ens_create(arr, len(self.computes), _native.ctypes.byref(h)))
         self._h = h
         self.tile = (4, 16)
-        self.fusion = (2, 1, 8)
+        self.fusion = (3, 1, 26)
+        self._shapes = {2: (1, 8), 3: (1, 26)}
         # the device table points at every member's buffers: a member may not
         # be closed while the ensemble exists (Compute.close raises)
         for c in self.computes:
@@ -83,19 +85,29 @@ class Ensemble:
         self.tile = (int(waves_x), int(rows))
 
     def set_fusion(self, depth, waves_x=0, rows=0):
-        """Iterations per batched launch (2 or 1) and the two-iteration
-        kernel's launch shape (<= 0 keeps the current one)."""
+        """Iterations per batched launch (3, 2 or 1) and the launch shape of
+        that depth's kernel (<= 0 keeps the current one)."""
         _native.check(self._lib.fd_ens_set_fusion(self._h, int(depth), int(waves_x), int(rows)))
-        wx = int(waves_x) if waves_x > 0 else self.fusion[1]
-        r = int(rows) if rows > 0 else self.fusion[2]
-        self.fusion = (int(depth), wx, r)
+        d = int(depth)
+        if d >= 2:
+            wx0, r0 = self._shapes[d]
+            self._shapes[d] = (int(waves_x) if waves_x > 0 else wx0, int(rows) if rows > 0 else r0)
+            self.fusion = (d,) + self._shapes[d]
+        else:
+            self.fusion = (1,) + self.fusion[1:]
+
+    @property
+    def depth(self):
+        """Iterations per batched launch that step() uses: the fusion depth
+        when every domain's geometry allows the fused kernels, else 1."""
+        ok = all(c.H >= 8 and (c.params.y_bc != Y_WALLS or (c.params.wall_lo == 0 and c.params.wall_hi == c.H - 1))
+                 for c in self.computes)
+        return self.fusion[0] if ok else 1
 
     @property
     def fused(self):
-        """True when pairs of iterations run as two-iteration launches."""
-        return self.fusion[0] == 2 and all(
-            c.H >= 8 and (c.params.y_bc != Y_WALLS or (c.params.wall_lo == 0 and c.params.wall_hi == c.H - 1))
-            for c in self.computes)
+        """True when groups of iterations run as two- or three-iteration launches."""
+        return self.depth >= 2
 
     def use_graph(self, on=True):
         _native.check(self._lib.fd_ens_use_graph(self._h, 1 if on else 0))
@@ -110,22 +122,22 @@ class Ensemble:
         """Time ``steps`` real iterations per launch shape of the kernel that
         runs (two-iteration when ``fused``) and keep the fastest.  Returns
         {shape: ms per iteration}."""
-        fused = self.fused
+        depth = self.depth
         if candidates is None:
-            candidates = FUSED_CANDIDATES if fused else TILE_CANDIDATES
-        steps += steps & 1
+            candidates = {3: FUSED3_CANDIDATES, 2: FUSED_CANDIDATES}.get(depth, TILE_CANDIDATES)
+        steps = max(6, steps - steps % 6)
         times = {}
         for wx, rows in candidates:
-            if fused:
-                self.set_fusion(2, wx, rows)
+            if depth >= 2:
+                self.set_fusion(depth, wx, rows)
             else:
                 self.set_tile(wx, rows)
-            self.step(4)                         # table rebuild and graph capture
+            self.step(6)                         # table rebuild and graph capture
             self.step(steps)
             times[(wx, rows)] = self.last_step_ms()
         best = min(times, key=times.get)
-        if fused:
-            self.set_fusion(2, *best)
+        if depth >= 2:
+            self.set_fusion(depth, *best)
         else:
             self.set_tile(*best)
         return times

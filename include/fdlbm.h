@@ -304,13 +304,15 @@ int fd_ens_step_sequential(fd_ensemble* e, int32_t nsteps);
  * (default 4 x 16). */
 int fd_ens_set_tile(fd_ensemble* e, int32_t waves_x, int32_t rows);
 
-/* Iterations per batched launch, as fd_set_fusion: depth 2 (the default)
- * advances pairs of iterations of every domain with one launch of the
- * two-iteration kernel, depth 1 one iteration per launch.  waves_x (1, 2 or
- * 4) and rows set the two-iteration launch shape (<= 0 keeps the current
- * one; default 1 x 8).  Pairs run batched only when every domain's geometry
- * allows the two-iteration kernel (fd_set_fusion); otherwise every launch
- * is of one iteration.  Results are identical bit for bit in every case.
+/* Iterations per batched launch, as fd_set_fusion: depth 3 (the default)
+ * advances triples of iterations of every domain with one launch of the
+ * three-iteration kernel, depth 2 pairs with the two-iteration kernel,
+ * depth 1 one iteration per launch.  waves_x and rows set that depth's
+ * launch shape (<= 0 keeps the current one; defaults 1 x 26 for depth 3,
+ * 1 x 8 for depth 2; depth 3 takes 1 or 2 waves).  Groups run batched only
+ * when every domain's geometry allows the fused kernels (fd_set_fusion);
+ * otherwise every launch is of one iteration.  Results are identical bit
+ * for bit in every case.
  * No reference counterpart (the reference runs one domain per process). */
 int fd_ens_set_fusion(fd_ensemble* e, int32_t depth, int32_t waves_x, int32_t rows);
 

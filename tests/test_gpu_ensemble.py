@@ -59,14 +59,14 @@ def _same(a, b):
     return True
 
 
-@pytest.mark.parametrize("depth", [2, 1])
+@pytest.mark.parametrize("depth", [3, 2, 1])
 def test_batched_equals_individual(depth):
     from fingering_dynamics_amd.ensemble import Ensemble
     ref = _solvers()
     cs = _solvers()
     ens = Ensemble(cs)
     ens.set_fusion(depth)
-    assert ens.fused == (depth == 2)
+    assert ens.depth == depth and ens.fused == (depth > 1)
     done = 0
     # graph replays (even chunks), odd chunks, eager launches, per-handle
     # reads between chunks, and a sequential stretch
@@ -102,10 +102,17 @@ def test_tile_independent_and_mass():
     for c in a:
         s_rho, s_psi, bad = c.mass()
         assert bad == 0 and np.isfinite(s_rho)
+    ea.set_fusion(2)
     times = ea.autotune(candidates=((4, 16), (2, 8)), steps=4)
     assert set(times) == {(4, 16), (2, 8)} and all(t > 0 for t in times.values())
     assert ea.fusion[1:] in times
     eb.step(a[0].t - b[0].t)         # the autotuner's trial iterations
+    assert _same(_state(a), _state(b))
+    # three iterations per launch
+    ea.set_fusion(3)
+    times = ea.autotune(candidates=((1, 16), (2, 5)), steps=6)
+    assert ea.fusion[1:] in times
+    eb.step(a[0].t - b[0].t)
     assert _same(_state(a), _state(b))
     # the same with one iteration per launch: the single-iteration shapes
     ea.set_fusion(1)
