@@ -974,6 +974,12 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         pc = lane_row(finish_g(k, ln, ys, flc, on, gc));
     }
     unsigned fla = flags_of(k, b, ln, ys + d);
+    // Stage A's pulls run one row ahead: the loads for row r + d are issued
+    // right after stage A of row r, so stages B and C of this row cover
+    // their latency.  The first row's are issued here.
+    float ga_n[9], f_n[9];
+    pull(src, k, 9, rows_of(k, wrap(ys + d, k.H)), po, fla & kLinks, true, ga_n);
+    pull(src, k, 0, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, true, f_n);
 
     // A -> B: g of A's rows r (Ga), r-d (Gb), r-2d (Gc); side values P of
     // the same rows.  Flags of rows r, r-d, ..., r-4d (ghost rows solid).
@@ -1017,9 +1023,12 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             // rewritten by Zou-He), and inert solid nodes hold zeros in
             // memory, which they keep; so no load needs a mask here
             float ga[9], f[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                ga[i] = ga_n[i];   // g pulled at row ya (issued one row ahead)
+                f[i] = f_n[i];     // f pulled at row r
+            }
             const bool ga_on = g_on(k, ln, ya, fla);
-            pull(src, k, 9, rows_of(k, wrap(ya, k.H)), po, fla & kLinks, true, ga);
-            pull(src, k, 0, rows_of(k, rw), po, flc & kLinks, true, f);
             const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, ga_on, ga));
             const PsiRow pn = d > 0 ? pa : pb;  // row r + 1
             const PsiRow ps = d > 0 ? pb : pa;  // row r - 1
@@ -1047,6 +1056,10 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             fr0 = arow && ln.inside ? flc : kSolid;
             flc = fla;
             fla = fl2;
+            // the next row's pulls (rows wrap, so the loads past the
+            // chunk's last row stay inside the allocation; they are unused)
+            pull(src, k, 9, rows_of(k, wrap(ya + d, k.H)), po, fla & kLinks, true, ga_n);
+            pull(src, k, 0, rows_of(k, wrap(r + d, k.H)), po, flc & kLinks, true, f_n);
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
