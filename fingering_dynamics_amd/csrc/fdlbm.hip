@@ -980,6 +980,11 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
     float ga_n[9], f_n[9];
     pull(src, k, 9, rows_of(k, wrap(ys + d, k.H)), po, fla & kLinks, true, ga_n);
     pull(src, k, 0, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, true, f_n);
+    // Stages B and C read their f rings one row ahead as well (the rows
+    // they read are complete one row earlier).
+    float fb_n[9], fc_n[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fb_n[i] = fc_n[i] = 0.f;
 
     // A -> B: g of A's rows r (Ga), r-d (Gb), r-2d (Gc); side values P of
     // the same rows.  Flags of rows r, r-d, ..., r-4d (ghost rows solid).
@@ -1101,7 +1106,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
                 const unsigned fl = fr2;
                 const bool solid = (fl & kSolid) != 0;
                 float f[9];
-                lds_pull(fring, y, lane, fl & kLinks, f);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) f[i] = fb_n[i];
                 float rho_nb = rho_prev;
                 if (has_in && brow) {
                     if (k.y_walls && y == (d > 0 ? 0 : k.H - 1))
@@ -1141,6 +1147,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             qyb = qya;
             qxa = qxn;
             qya = qyn;
+            // B's pull of the next row, r - d: rows r - 2d .. r are in the ring
+            lds_pull(fring, r - d, lane, fr1 & kLinks, fb_n);
         }
         if (j < 6) continue;
         // ---------------- stage C: psi_n+3 and g of row r-3d
@@ -1166,7 +1174,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             const unsigned fl = fr4;
             const bool solid = (fl & kSolid) != 0;
             float f[9];
-            lds_pull(fring2, y, lane, fl & kLinks, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) f[i] = fc_n[i];
             float rho_nb = rho_prev_c;
             if (has_in) {
                 if (k.y_walls && y == (d > 0 ? 0 : k.H - 1))
@@ -1202,6 +1211,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         uc = un;
 #pragma unroll
         for (int i = 0; i < 9; ++i) gu[i] = gv[i];
+        // C's pull of the next row, r - 3d: B wrote row r - 2d above
+        lds_pull(fring2, r - 3 * d, lane, fr3 & kLinks, fc_n);
     }
 }
 
