@@ -78,7 +78,7 @@ def test_fused_equals_single_on_the_parity_cases(case, depth):
 
 
 @pytest.mark.parametrize("depth", [2, 3])
-@pytest.mark.parametrize("W", [8, 37, 53, 54, 55, 57, 58, 59, 61, 64, 107, 108, 109, 115, 116, 117, 121, 131, 250])
+@pytest.mark.parametrize("W", [8, 37, 53, 54, 55, 57, 58, 59, 61, 64, 107, 108, 109, 113, 114, 115, 116, 117, 121, 131, 250])
 @pytest.mark.parametrize("periodic_x", [False, True])
 def test_fused_equals_single_across_widths(W, periodic_x, depth):
     H = 20
