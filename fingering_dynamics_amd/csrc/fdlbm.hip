@@ -1823,9 +1823,11 @@ int fused3_slots(int dev, const hipDeviceProp_t& prop, int* slots) {
 // launches of fewer than three rounds 15 % dearer (their waves do not
 // finish together); the chosen height is the one with the least such cost
 // over 8..48 rows (the smallest on a tie).  Measured on 4096 x 2048 (76
-// strips, 2048 slots), ms per iteration: 26 rows (6,004 waves, 2.93
-// rounds) 0.164, 39 rows (1.97 rounds) 0.167, 20 rows (3.82) 0.170, 25
-// rows (3.04) 0.175, 52 rows (1.48) 0.187.
+// strips, 2048 slots) with an earlier build of the kernel, ms per
+// iteration: 26 rows (6,004 waves, 2.93 rounds) 0.164, 39 rows (1.97
+// rounds) 0.167, 20 rows (3.82) 0.170, 25 rows (3.04) 0.175, 52 rows
+// (1.48) 0.187; with the current one the minima are flatter (16 rows
+// 0.148, 20 rows 0.149, 26 rows 0.149, 24 rows 0.155).
 int fused3_rows(int W, int H, int slots) {
     const int64_t strips = (W + kFuse3Cols - 1) / kFuse3Cols;
     int best = 8;
@@ -2085,7 +2087,7 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
         h->f3rows = fused3_rows(p->W, p->H, slots);
         // three iterations per launch on domains that fill the device at
         // least twice over with waves of >= 16 rows (measured on 4096 x 2048:
-        // 0.164 ms per iteration at 1 x 26 against 0.175 with two)
+        // 0.149 ms per iteration at 1 x 26 against 0.175 with two)
         if (geom_fusable(*p) && h->f3rows >= 16 &&
             (int64_t)((p->W + kFuse3Cols - 1) / kFuse3Cols) * ((p->H + h->f3rows - 1) / h->f3rows) >= 2 * (int64_t)slots)
             h->fuse = 3;
