@@ -922,6 +922,13 @@ __global__ __launch_bounds__(64 * WPB) void fused_kernel(KParams k, KBufs b) {
 // launches bit for bit.
 constexpr int kFuse3Cols = 54;
 
+// Rows per wave of the three-iteration kernel.  Measured on 4096 x 2048 (76
+// strips, 2,048 concurrent waves), ms per iteration: 12 rows 0.153, 14
+// 0.151, 15 0.150, 16 0.1455, 17 0.148, 18 0.148, 20 0.148, 22 0.152, 24
+// 0.155, 26 0.154, 32 0.156.  Taller chunks save stage A's 8 extra rows per
+// chunk but run slower per row; shorter ones pay more extra rows.
+constexpr int kFuse3Rows = 16;
+
 // rings_all: the workgroup's LDS rings, WPB * 2 * kRingRegion floats,
 // declared by the kernel (one array whatever copies of the march it holds).
 template <int WPB, bool EDGE>
@@ -982,6 +989,12 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         pc = lane_row(finish_g(k, ln, ys, flc, EDGE && on, gc));
     }
     unsigned fla = flags_of(k, b, ln, ys + d);
+    // Flags are loaded two rows ahead: the pulls issued after stage A need
+    // the flags of row r + 2d, and a flag load issued at the top of the same
+    // step would be younger than stage C's stores of the step before, which
+    // the wait for it would then cover too (0.1543 -> 0.1455 ms per iteration
+    // on 4096 x 2048 at 1 x 16).
+    unsigned flb = flags_of(k, b, ln, ys + 2 * d);
     // Stage A's pulls run one row ahead: the loads for row r + d are issued
     // right after stage A of row r, so stages B and C of this row cover
     // their latency.  The first row's are issued here.
@@ -1028,7 +1041,7 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         float Gn[9], pxn = 0.f, pyn = 0.f;
         {
             const int ya = r + d;
-            const unsigned fl2 = flags_of(k, b, ln, ya + d);
+            const unsigned fl3 = flags_of(k, b, ln, ya + 2 * d);
             // every lane loads: what inert solids, ghost rows and lanes
             // beyond the domain load is never used by a node of the domain
             // (psi and the collision of those lanes do not depend on it, and
@@ -1068,7 +1081,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             fr1 = fr0;
             fr0 = arow && ln.inside ? flc : kSolid;
             flc = fla;
-            fla = fl2;
+            fla = flb;
+            flb = fl3;
             // the next row's pulls (rows wrap, so the loads past the
             // chunk's last row stay inside the allocation; they are unused)
             pull(src, k, 9, rows_of(k, wrap(ya + d, k.H)), po, fla & kLinks, true, ga_n);
@@ -1587,7 +1601,7 @@ struct fd_handle {
     int fwx = 1;    // fused_kernel: waves per workgroup
     int frows = 12; // fused_kernel: output rows per wave
     int f3wx = 1;   // fused3_kernel: waves per workgroup
-    int f3rows = 20; // fused3_kernel: output rows per wave
+    int f3rows = kFuse3Rows; // fused3_kernel: output rows per wave
     int timed_launches = 0;
     bool have_timing = false;
     int ens_refs = 0;  // ensembles holding this handle: fd_destroy refuses while > 0
@@ -1814,34 +1828,6 @@ int fused3_slots(int dev, const hipDeviceProp_t& prop, int* slots) {
     (void)dev;
     *slots = std::max(1, per_cu) * std::max(1, prop.multiProcessorCount);
     return FD_OK;
-}
-
-// Rows per wave of the three-iteration kernel on an H x W domain with
-// `slots` concurrent waves.  Its waves are bound by their own arithmetic
-// and run in rounds of `slots`: a launch costs about rounds x (rows + 8)
-// stage-A rows (the 8 rows beyond the chunk that stages A and B add), with
-// launches of fewer than three rounds 15 % dearer (their waves do not
-// finish together); the chosen height is the one with the least such cost
-// over 8..48 rows (the smallest on a tie).  Measured on 4096 x 2048 (76
-// strips, 2048 slots) with an earlier build of the kernel, ms per
-// iteration: 26 rows (6,004 waves, 2.93 rounds) 0.164, 39 rows (1.97
-// rounds) 0.167, 20 rows (3.82) 0.170, 25 rows (3.04) 0.175, 52 rows
-// (1.48) 0.187; with the current one the minima are flatter (16 rows
-// 0.148, 20 rows 0.149, 26 rows 0.149, 24 rows 0.155).
-int fused3_rows(int W, int H, int slots) {
-    const int64_t strips = (W + kFuse3Cols - 1) / kFuse3Cols;
-    int best = 8;
-    double best_cost = -1.0;
-    for (int rows = 8; rows <= 48; ++rows) {
-        const int64_t waves = strips * ((H + rows - 1) / rows);
-        const int64_t rounds = (waves + slots - 1) / slots;
-        const double cost = (double)(rounds * (rows + 8)) * (rounds < 3 ? 1.15 : 1.0);
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best = rows;
-        }
-    }
-    return best;
 }
 
 // One launch of the three-iteration kernel (one or two waves per
@@ -2084,11 +2070,11 @@ int fd_create(const fd_params* p, const uint8_t* solid, const uint8_t* links, fd
             delete h;
             return rc;
         }
-        h->f3rows = fused3_rows(p->W, p->H, slots);
+        h->f3rows = kFuse3Rows;
         // three iterations per launch on domains that fill the device at
-        // least twice over with waves of >= 16 rows (measured on 4096 x 2048:
-        // 0.149 ms per iteration at 1 x 26 against 0.175 with two)
-        if (geom_fusable(*p) && h->f3rows >= 16 &&
+        // least twice over with waves of 16 rows (measured on 4096 x 2048:
+        // 0.1455 ms per iteration at 1 x 16 against 0.175 with two)
+        if (geom_fusable(*p) &&
             (int64_t)((p->W + kFuse3Cols - 1) / kFuse3Cols) * ((p->H + h->f3rows - 1) / h->f3rows) >= 2 * (int64_t)slots)
             h->fuse = 3;
     }

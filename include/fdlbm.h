@@ -237,9 +237,9 @@ int fd_set_tile(fd_handle* h, int32_t waves_x, int32_t rows);
  * for triples (two intermediate states on chip); depth 1 runs one launch
  * per iteration.  The results are identical bit for bit.  waves_x (1, 2 or
  * 4; depth 3 uses 1 or 2) and rows set the launch shape of that depth's
- * kernel (<= 0 keeps the current one).  Defaults: depth 3 at a height
- * chosen for whole launch rounds on domains that fill the device at least
- * twice over (26 rows on 4096 x 2048), else depth 2 at 1 wave x 8 rows,
+ * kernel (<= 0 keeps the current one).  Defaults: depth 3 at 1 wave x 16
+ * rows on domains that fill the device at least twice over with such waves
+ * (4096 x 2048 does), else depth 2 at 1 wave x 8 rows,
  * fewer rows on domains too small to fill the device.  Domains with H < 8,
  * and walled domains whose walls are not the edge rows (the rows 1 and H-2
  * of fingering.py:573), always run depth 1.  No reference counterpart (a
