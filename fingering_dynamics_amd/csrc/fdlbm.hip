@@ -29,6 +29,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fdlbm.h"
@@ -973,20 +974,24 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
     const int yfc = ys + 4 * d;               // stage C's first row
 
     // ---- stage A prologue: psi_n+1 of rows ys-d, ys and g of row ys
-    float gc[9];
+    // GA: g pulled by stage A, two slots: at a step of phase P (= j mod 2),
+    // slot P holds row r (for the collision) and slot 1-P row r+d (for psi);
+    // the pulls of row r+2d go to slot P once its row r is consumed, so the
+    // loads in flight are never moved between registers.
+    float GA[2][9];
     PsiRow pb, pc;
     {
         const int yb = ys - d;
         const unsigned fb = flags_of(k, b, ln, yb);
         const bool on = g_on(k, ln, yb, fb);
-        pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, gc);
-        pb = lane_row(finish_g(k, ln, yb, fb, EDGE && on, gc));
+        pull(src, k, 9, rows_of(k, wrap(yb, k.H)), po, fb & kLinks, on, GA[0]);
+        pb = lane_row(finish_g(k, ln, yb, fb, EDGE && on, GA[0]));
     }
     unsigned flc = flags_of(k, b, ln, ys);
     {
         const bool on = g_on(k, ln, ys, flc);
-        pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, gc);
-        pc = lane_row(finish_g(k, ln, ys, flc, EDGE && on, gc));
+        pull(src, k, 9, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, on, GA[0]);
+        pc = lane_row(finish_g(k, ln, ys, flc, EDGE && on, GA[0]));
     }
     unsigned fla = flags_of(k, b, ln, ys + d);
     // Flags are loaded two rows ahead: the pulls issued after stage A need
@@ -998,8 +1003,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
     // Stage A's pulls run one row ahead: the loads for row r + d are issued
     // right after stage A of row r, so stages B and C of this row cover
     // their latency.  The first row's are issued here.
-    float ga_n[9], f_n[9];
-    pull(src, k, 9, rows_of(k, wrap(ys + d, k.H)), po, fla & kLinks, true, ga_n);
+    float f_n[9];
+    pull(src, k, 9, rows_of(k, wrap(ys + d, k.H)), po, fla & kLinks, true, GA[1]);
     pull(src, k, 0, rows_of(k, wrap(ys, k.H)), po, flc & kLinks, true, f_n);
     // Stages B and C read their f rings one row ahead as well (the rows
     // they read are complete one row earlier).
@@ -1033,7 +1038,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
     float rho_prev_c = 0.f;
 
     const int nj = (y1 - y0) + 8;
-    for (int j = 0; j < nj; ++j) {
+    auto step = [&](auto phase, const int j) __attribute__((always_inline)) {
+        constexpr int P = decltype(phase)::value;
         const int r = ys + j * d;    // stage A row
         const bool arow = !(k.y_walls && (r < 0 || r >= k.H));
         const int rw = wrap(r, k.H);
@@ -1048,12 +1054,8 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             // every pull of a domain node from them is a bounce-back link or
             // rewritten by Zou-He), and inert solid nodes hold zeros in
             // memory, which they keep; so no load needs a mask here
-            float ga[9], f[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                ga[i] = ga_n[i];   // g pulled at row ya (issued one row ahead)
-                f[i] = f_n[i];     // f pulled at row r
-            }
+            float (&ga)[9] = GA[1 - P];   // g pulled at row ya (issued one row ahead)
+            float (&f)[9] = f_n;          // f pulled at row r
             const bool ga_on = g_on(k, ln, ya, fla);
             const PsiRow pa = lane_row(finish_g(k, ln, ya, fla, EDGE && ga_on, ga));
             const PsiRow pn = d > 0 ? pa : pb;  // row r + 1
@@ -1063,7 +1065,7 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             fdn::stencil(nb9, dpx, dpy, lap);
             if (EDGE && arow) bc_f(k, b, src, rw, (int)ln.x, f);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Gn[i] = gc[i];
+            for (int i = 0; i < 9; ++i) Gn[i] = GA[P][i];
             const bool solid = !arow || !ln.inside || (flc & kSolid) != 0;
             const float mu = collide_node(k, f, Gn, pc.c, dpx, dpy, lap, solid);
             float* const slot = fring + (r & 3) * kRingSlot + lane;
@@ -1073,8 +1075,6 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             pyn = dpy * mu;
             pb = pc;
             pc = pa;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) gc[i] = ga[i];
             fr4 = fr3;
             fr3 = fr2;
             fr2 = fr1;
@@ -1085,7 +1085,7 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             flb = fl3;
             // the next row's pulls (rows wrap, so the loads past the
             // chunk's last row stay inside the allocation; they are unused)
-            pull(src, k, 9, rows_of(k, wrap(ya + d, k.H)), po, fla & kLinks, true, ga_n);
+            pull(src, k, 9, rows_of(k, wrap(ya + d, k.H)), po, fla & kLinks, true, GA[P]);
             pull(src, k, 0, rows_of(k, wrap(r + d, k.H)), po, flc & kLinks, true, f_n);
         }
 #pragma unroll
@@ -1100,7 +1100,7 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         pyb = pya;
         pxa = pxn;
         pya = pyn;
-        if (j < 2) continue;
+        if (j < 2) return;
         // ---------------- stage B: psi_n+2 and g of row r-d
         {
             PsiRow qn;
@@ -1172,7 +1172,7 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
             // B's pull of the next row, r - d: rows r - 2d .. r are in the ring
             lds_pull(fring, r - d, lane, fr1 & kLinks, fb_n);
         }
-        if (j < 6) continue;
+        if (j < 6) return;
         // ---------------- stage C: psi_n+3 and g of row r-3d
         PsiRow un;
         float gv[9];
@@ -1235,6 +1235,10 @@ __device__ __forceinline__ void fused3_march(const KParams& k, const KBufs& b, i
         for (int i = 0; i < 9; ++i) gu[i] = gv[i];
         // C's pull of the next row, r - 3d: B wrote row r - 2d above
         lds_pull(fring2, r - 3 * d, lane, fr3 & kLinks, fc_n);
+    };
+    for (int j = 0; j < nj; j += 2) {
+        step(std::integral_constant<int, 0>(), j);
+        if (j + 1 < nj) step(std::integral_constant<int, 1>(), j + 1);
     }
 }
 
