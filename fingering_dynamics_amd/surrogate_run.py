@@ -107,7 +107,8 @@ def main(argv=None):
     ap.add_argument("--mfma-only", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="accelerated run only")
     ap.add_argument("--variants", default="",
-                    help="further accelerated runs as relax:velocity-scale:band, e.g. 500:0:2,1000:0.5:2")
+                    help="further accelerated runs as relax[:velocity-scale[:band[:threshold]]], "
+                         "e.g. 500:0:2,1000:0.5:2,200:0:2:0.005")
     ap.add_argument("--band", type=int, default=2, help="coarse cells around the interface a jump may change")
     ap.add_argument("--dagger-cycles", type=int, default=0,
                     help="coupled-loop cycles on the training flows whose states, labelled by the solver, "
@@ -202,24 +203,26 @@ def main(argv=None):
     model.eval()
     fluid = torch.as_tensor(mask, dtype=torch.float32).cuda()
     fluid_c = (fluid.view(spec["H"] // a.f, a.f, spec["W"] // a.f, a.f).sum(dim=(1, 3)) > 0).float()
-    runs = [(a.relax, a.velocity, a.band)]
+    runs = [(a.relax, a.velocity, a.band, a.threshold)]
     for v in a.variants.split(","):
         if v:
-            r, s, b = (v.split(":") + [str(a.band)])[:3]
-            runs.append((int(r), float(s), int(b)))
+            f = v.split(":")
+            f += [str(a.velocity), str(a.band), str(a.threshold)][len(f) - 1:]
+            runs.append((int(f[0]), float(f[1]), int(f[2]), float(f[3])))
 
     def run_accelerated(net):
         out = []
-        for relax, vel, band in runs:
+        for relax, vel, band, threshold in runs:
             # the plain run's coarse snapshots sit at chunk + k * (jump + relax)
             # only for the main relax length; other variants compare at breakthrough
             cm = Compute(mask, links, params)
             t_a, wall_a, stats, rec_a = sg.accelerated_run(cm, net, spec["M"], 180.0 - spec["theta"], a.f, a.jump,
-                                                           relax, a.threshold, a.max_steps,
+                                                           relax, threshold, a.max_steps,
                                                            mass_fix=not a.no_mass_fix, velocity=vel, band=band,
                                                            teacher=rec_p if a.teacher else None,
                                                            teacher_velocity=rec_u, log=log)
-            acc = dict(stats, relax=relax, velocity=vel, band=band, breakthrough_t=t_a, wall_s=wall_a,
+            acc = dict(stats, relax=relax, velocity=vel, band=band, threshold=threshold, breakthrough_t=t_a,
+                       wall_s=wall_a,
                        non_finite=cm.mass()[2])
             cm.close()
             if "plain" in res:
