@@ -85,6 +85,35 @@ def mfma_throughput(model, H=128, W=256, bs=64, iters=20):
             "frac": tf / BF16_PEAK_TFLOPS, "gflop_per_step": flops / 1e9}
 
 
+def select_network(a, nets, seeds, log=print):
+    """Model selection on a validation bed: the breakthrough workload's
+    flow with obstacle seed a.select_workload_seed (a different bed from the
+    target's), run plainly and accelerated by each network with the main
+    jump, relax, band and threshold; the network whose breakthrough comes
+    closest to the plain run's is selected."""
+    spec = dict(sg.breakthrough_spec(), obstacle_seed=a.select_workload_seed)
+    mask, links, params = sg.build_flow(spec)
+    cm = Compute(mask, links, params)
+    t_p, wall_p, _ = sg.plain_run(cm, a.f, a.max_steps, chunk=a.relax, record_every=a.jump + a.relax, log=log)
+    cm.close()
+    out = {"workload_seed": a.select_workload_seed, "plain": {"breakthrough_t": t_p, "wall_s": wall_p}, "runs": []}
+    best, best_err = 0, None
+    for k, (net, sd) in enumerate(zip(nets, seeds)):
+        cm = Compute(mask, links, params)
+        t_a, wall_a, stats, _ = sg.accelerated_run(cm, net, spec["M"], 180.0 - spec["theta"], a.f, a.jump, a.relax,
+                                                   a.threshold, a.max_steps, mass_fix=not a.no_mass_fix,
+                                                   velocity=a.velocity, band=a.band, log=log)
+        cm.close()
+        err = (t_a - t_p) / t_p if (t_a and t_p) else None
+        out["runs"].append({"seed": sd, "breakthrough_t": t_a, "wall_s": wall_a, "rel_error": err,
+                            "speedup_wall": wall_p / wall_a if wall_a > 0 else None})
+        log("selection: seed %d breakthrough %s (plain %s)" % (sd, t_a, t_p))
+        if err is not None and (best_err is None or abs(err) < best_err):
+            best, best_err = k, abs(err)
+    out["selected_index"], out["selected_seed"] = best, seeds[best]
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -121,6 +150,10 @@ def main(argv=None):
     ap.add_argument("--extra-seeds", default="",
                     help="further networks (initialisation and batch order seeds, e.g. 1,2,3) trained on the "
                          "same data and run through the same accelerated runs: the spread over training")
+    ap.add_argument("--select-workload-seed", type=int, default=None,
+                    help="choose, among the networks trained (the first and --extra-seeds), the one whose "
+                         "breakthrough on the workload with this obstacle seed is closest to that workload's "
+                         "plain run (a validation bed, not the target), and report its target run as 'selected'")
     ap.add_argument("--ensemble-eval", action="store_true",
                     help="with --extra-seeds: also run the accelerated runs with the mean of all the networks")
     ap.add_argument("--workload-seed", type=int, default=None,
@@ -249,6 +282,12 @@ def main(argv=None):
         nets.append(net)
         res.setdefault("seeds", []).append({"seed": sd, "train_seconds": secs_s, "eval_heldout": ev,
                                             "accelerated": run_accelerated(net)})
+    if a.select_workload_seed is not None:
+        res["selection"] = select_network(a, nets, [0] + [int(v) for v in a.extra_seeds.split(",") if v], log)
+        k = res["selection"]["selected_index"]
+        res["selected"] = dict(seed=res["selection"]["selected_seed"],
+                               accelerated=(res["accelerated"] if k == 0 else res["seeds"][k - 1]["accelerated"])[0])
+        log("selected: %s" % json.dumps(res["selected"], default=float))
     if a.ensemble_eval and len(nets) > 1:
         mean = MeanNet(nets).eval()
         res["ensemble"] = {"networks": len(nets), "eval_heldout": sg.evaluate(mean, data, te, a.jump, a.snap),
