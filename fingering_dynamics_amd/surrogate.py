@@ -45,6 +45,8 @@ U_REF = 1e-3          # velocity scale of the inputs (the workload's inlet veloc
 LAMBDA_U = 0.01       # weight of the velocity change in the training loss
 LAMBDA_S = 0.0        # weight of the phase (sign) cross-entropy on interface cells (0.01 tried)
 SIGN_GAIN = 4.0       # logit = SIGN_GAIN * predicted psi
+LAMBDA_V = 0.0        # weight of the squared error of each sample's total psi change (the volume the jump
+                      # adds), relative to the batch's mean true total (0.03 tried)
 LOSS_BAND = 0         # > 0: the psi and velocity MSE count only cells within this many coarse cells of
                       # the interface at t, the cells a jump may change (0: every fluid cell)
 CHANNELS = (32, 64, 128, 128)
@@ -382,11 +384,18 @@ def loss_fn(model, x, psi0, psi1, v0, v1, wfluid):
     lp = ((pred - psi1) ** 2 * wfluid).sum() / w
     lu = (((v0 + out[..., 1:3] - v1) ** 2).sum(dim=-1) * wfluid).sum() / w
     band = ((psi0.abs() < 0.9) | (psi1.abs() < 0.9)).float() * wfluid
+    loss = lp + LAMBDA_U * lu
+    if LAMBDA_V > 0.0:
+        # networks whose jumps add too little volume lag in the coupled run
+        # (the mass correction scales their smeared increments up instead)
+        gp = (out[..., 0] * wfluid).sum(dim=(1, 2))
+        gt = ((psi1 - psi0) * wfluid).sum(dim=(1, 2))
+        loss = loss + LAMBDA_V * (((gp - gt) / gt.abs().mean().clamp(min=1e-6)) ** 2).mean()
     if LAMBDA_S == 0.0:
-        return lp + LAMBDA_U * lu
+        return loss
     ls = torch.nn.functional.binary_cross_entropy_with_logits(
         SIGN_GAIN * pred, (psi1 > 0).float(), weight=band, reduction="sum") / band.sum().clamp(min=1.0)
-    return lp + LAMBDA_U * lu + LAMBDA_S * ls
+    return loss + LAMBDA_S * ls
 
 
 def train(model, data, train_idx, jump, snap, iters=2000, bs=64, lr=1e-3, wd=1e-4, seed=0, extra=None,

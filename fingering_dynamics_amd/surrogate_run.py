@@ -160,6 +160,8 @@ def main(argv=None):
                     help="obstacle seed of the 2048 x 1024 workload (default 5)")
     ap.add_argument("--phase-weight", type=float, default=sg.LAMBDA_S,
                     help="weight of the phase cross-entropy term of the training loss")
+    ap.add_argument("--volume-weight", type=float, default=sg.LAMBDA_V,
+                    help="weight of each sample's total psi-change error in the training loss")
     ap.add_argument("--loss-band", type=int, default=sg.LOSS_BAND,
                     help="train on cells within this many coarse cells of the interface only (0: all)")
     ap.add_argument("--regime", default="target", choices=sorted(sg.REGIMES))
@@ -170,6 +172,7 @@ def main(argv=None):
         ap.error("--extra-seeds trains on generated data; it cannot be combined with --load-model")
     sg.LAMBDA_S = a.phase_weight
     sg.LOSS_BAND = a.loss_band
+    sg.LAMBDA_V = a.volume_weight
     os.makedirs(a.out, exist_ok=True)
     res = {"args": vars(a)}
     model = sg.UNet(seed=0).cuda()

@@ -226,3 +226,44 @@ def test_ensemble_domain_specs_and_square_beds():
     assert p.H == spec["H"] and p.W == spec["W"] and p.M == spec["M"] and p.u0 == 0.001
     assert p.y_bc == 0 and p.psi_visc_bound == 1.0      # periodic y, |psi| bound in tau_mix
     assert p.psi_wall == wetting.psi_wall_for_angle(180.0 - spec["theta"])
+
+
+def test_volume_term_of_the_loss():
+    """The volume term adds LAMBDA_V times the squared error of each
+    sample's total psi change, relative to the batch's mean true total: zero
+    when every sample's predicted total equals the true one (whatever the
+    per-cell errors), and quadratic in a uniform under-prediction."""
+
+    class Const(torch.nn.Module):
+        def __init__(self, out):
+            super().__init__()
+            self.out = out
+
+        def forward(self, x):
+            return self.out
+
+    B, H, W = 2, 4, 6
+    x = torch.zeros(B, H, W, sg.IN_CH)
+    p0 = torch.zeros(B, H, W)
+    p1 = torch.full((B, H, W), 0.02)
+    v = torch.zeros(B, H, W, 2)
+    wf = torch.ones(B, H, W)
+    old = sg.LAMBDA_V
+    try:
+        # same totals, different cells: the volume term is zero
+        out = torch.zeros(B, H, W, 3)
+        out[:, 0, :, 0] = 0.02 * H
+        sg.LAMBDA_V = 0.0
+        base = float(sg.loss_fn(Const(out), x, p0, p1, v, v, wf))
+        sg.LAMBDA_V = 0.5
+        assert math.isclose(float(sg.loss_fn(Const(out), x, p0, p1, v, v, wf)), base, rel_tol=1e-6)
+        # a uniform under-prediction by a fraction e adds LAMBDA_V * e^2
+        for e in (0.25, 0.5):
+            out = torch.zeros(B, H, W, 3)
+            out[..., 0] = 0.02 * (1 - e)
+            sg.LAMBDA_V = 0.0
+            base = float(sg.loss_fn(Const(out), x, p0, p1, v, v, wf))
+            sg.LAMBDA_V = 0.5
+            assert math.isclose(float(sg.loss_fn(Const(out), x, p0, p1, v, v, wf)) - base, 0.5 * e * e, rel_tol=1e-5)
+    finally:
+        sg.LAMBDA_V = old
