@@ -556,6 +556,89 @@ __global__ __launch_bounds__(256) void reduce_slices(const float* __restrict__ p
     out[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
+// ------------------------------------------------------ U-Net plumbing
+// The decoder input: out[n][h][w][0:C1] = lo[n][h/2][w/2][:] (nearest-
+// neighbour upsampling of the coarser level) and out[n][h][w][C1:C1+C2] =
+// skip[n][h][w][:] (the skip connection), one pass of 16-byte pieces of 8
+// channels (G1 = C1 / 8, G2 = C2 / 8 pieces per pixel); n pieces in all.
+__global__ __launch_bounds__(256) void up_cat(const uint16_t* __restrict__ lo, const uint16_t* __restrict__ skip,
+                                              uint16_t* __restrict__ out, int H, int W, int G1, int G2, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int G = G1 + G2;
+    const int64_t p = i / G;
+    const int g = (int)(i - p * G);
+    u4v v;
+    if (g < G1) {
+        const int w = (int)(p % W);
+        const int64_t t = p / W;
+        const int h = (int)(t % H);
+        const int64_t nn = t / H;
+        v = reinterpret_cast<const u4v*>(lo)[((nn * (H / 2) + h / 2) * (W / 2) + w / 2) * G1 + g];
+    } else {
+        v = reinterpret_cast<const u4v*>(skip)[p * G2 + (g - G1)];
+    }
+    reinterpret_cast<u4v*>(out)[i] = v;
+}
+
+__device__ __forceinline__ float bf_lo(unsigned v) { return __builtin_bit_cast(float, v << 16); }
+__device__ __forceinline__ float bf_hi(unsigned v) { return __builtin_bit_cast(float, v & 0xFFFF0000u); }
+
+// Backward of up_cat: dlo[n][h][w][c] = the sum of dout[n][2h+a][2w+b][c]
+// over the 2 x 2 block, taken in the order of the gradients of the two
+// repeats (along w, then along h): the two columns of each row summed and
+// rounded to bf16, then the two rows summed and rounded; dskip =
+// dout[..., C1:].  Pieces [0, n_lo) are dlo's, [n_lo, n_lo + n_skip) dskip's.
+__global__ __launch_bounds__(256) void up_cat_bwd(const uint16_t* __restrict__ dout, uint16_t* __restrict__ dlo,
+                                                  uint16_t* __restrict__ dskip, int H, int W, int G1, int G2,
+                                                  int64_t n_lo, int64_t n_skip) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = G1 + G2;
+    const u4v* d = reinterpret_cast<const u4v*>(dout);
+    if (i < n_lo) {
+        const int64_t q = i / G1;
+        const int g = (int)(i - q * G1);
+        const int w2 = W / 2, h2 = H / 2;
+        const int w = (int)(q % w2);
+        const int64_t t = q / w2;
+        const int h = (int)(t % h2);
+        const int64_t nn = t / h2;
+        const int64_t p00 = (nn * H + 2 * h) * W + 2 * w;
+        const u4v a = d[p00 * G + g], b = d[(p00 + 1) * G + g];
+        const u4v c = d[(p00 + W) * G + g], e = d[(p00 + W + 1) * G + g];
+        u4v r;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float t0 = bf_lo(to_bf16(bf_lo(a[k]) + bf_lo(b[k]))), t1 = bf_lo(to_bf16(bf_lo(c[k]) + bf_lo(e[k])));
+            const float u0 = bf_lo(to_bf16(bf_hi(a[k]) + bf_hi(b[k]))), u1 = bf_lo(to_bf16(bf_hi(c[k]) + bf_hi(e[k])));
+            r[k] = (unsigned)to_bf16(t0 + t1) | ((unsigned)to_bf16(u0 + u1) << 16);
+        }
+        reinterpret_cast<u4v*>(dlo)[i] = r;
+    } else if (i < n_lo + n_skip) {
+        const int64_t j = i - n_lo;
+        const int64_t p = j / G2;
+        const int g = (int)(j - p * G2);
+        reinterpret_cast<u4v*>(dskip)[j] = d[p * G + G1 + g];
+    }
+}
+
+// out = dy where y > 0 (as a float: -0 and NaN are not), else 0: the
+// backward mask of a ReLU whose bf16 output is y; n8 pieces of 8 values.
+__global__ __launch_bounds__(256) void relu_mask(const uint16_t* __restrict__ dy, const uint16_t* __restrict__ y,
+                                                 uint16_t* __restrict__ out, int64_t n8) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const u4v v = reinterpret_cast<const u4v*>(dy)[i], m = reinterpret_cast<const u4v*>(y)[i];
+    u4v r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned lo = bf_lo(m[k]) > 0.f ? 0x0000FFFFu : 0u;
+        const unsigned hi = bf_hi(m[k]) > 0.f ? 0xFFFF0000u : 0u;
+        r[k] = v[k] & (lo | hi);
+    }
+    reinterpret_cast<u4v*>(out)[i] = r;
+}
+
 int check(int N, int H, int W, int Cin, int Cout) {
     if (N < 1 || H < 1 || W < 1 || Cin < 8 || Cout < 1) return fail(FDS_E_ARG, "bad shape");
     if (Cin % 8) return fail(FDS_E_ARG, "Cin must be a multiple of 8");
@@ -715,6 +798,39 @@ int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t
                            (int64_t)Cout, S);
         FDS_HIP(hipGetLastError());
     }
+    return FDS_OK;
+}
+
+int fds_up_cat(const uint16_t* lo, const uint16_t* skip, uint16_t* out, int32_t N, int32_t H, int32_t W,
+               int32_t C1, int32_t C2, void* stream) {
+    if (!lo || !skip || !out) return fail(FDS_E_ARG, "NULL pointer");
+    if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
+    if (C1 < 8 || C2 < 8 || (C1 % 8) || (C2 % 8)) return fail(FDS_E_ARG, "C1, C2 must be positive multiples of 8");
+    const int64_t n = (int64_t)N * H * W * ((C1 + C2) / 8);
+    hipLaunchKernelGGL(up_cat, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, lo, skip, out, H,
+                       W, C1 / 8, C2 / 8, n);
+    FDS_HIP(hipGetLastError());
+    return FDS_OK;
+}
+
+int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t N, int32_t H, int32_t W, int32_t C1,
+                   int32_t C2, void* stream) {
+    if (!dout || !dlo || !dskip) return fail(FDS_E_ARG, "NULL pointer");
+    if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
+    if (C1 < 8 || C2 < 8 || (C1 % 8) || (C2 % 8)) return fail(FDS_E_ARG, "C1, C2 must be positive multiples of 8");
+    const int64_t n_lo = (int64_t)N * (H / 2) * (W / 2) * (C1 / 8), n_skip = (int64_t)N * H * W * (C2 / 8);
+    hipLaunchKernelGGL(up_cat_bwd, dim3((unsigned)((n_lo + n_skip + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       dout, dlo, dskip, H, W, C1 / 8, C2 / 8, n_lo, n_skip);
+    FDS_HIP(hipGetLastError());
+    return FDS_OK;
+}
+
+int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t n, void* stream) {
+    if (!dy || !y || !out) return fail(FDS_E_ARG, "NULL pointer");
+    if (n < 8 || (n % 8)) return fail(FDS_E_ARG, "n must be a positive multiple of 8");
+    hipLaunchKernelGGL(relu_mask, dim3((unsigned)((n / 8 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dy, y,
+                       out, n / 8);
+    FDS_HIP(hipGetLastError());
     return FDS_OK;
 }
 

@@ -12,9 +12,10 @@ down, nearest-neighbour upsampling and skip concatenation up, ReLU after
 every convolution but the head: about 1.2M parameters.  Every convolution
 (forward, input gradient, weight and bias gradients) runs on the BF16 MFMA
 kernels of libfdsurr.so with FP32 accumulation; activations are bf16 NHWC,
-master weights FP32 with AdamW.  Pooling, upsampling, concatenation, the
-ReLU mask and the optimiser are PyTorch tensor operations (plumbing around
-the convolutions).
+master weights FP32 with AdamW.  The decoder's upsampling + skip
+concatenation and the ReLU backward mask run on small kernels of the same
+library (bit for bit the PyTorch composition they replace); pooling and the
+optimiser are PyTorch tensor operations (plumbing around the convolutions).
 
 Data.  Seeded beds of circles (flow_spec, build_flow) at a fixed size,
 advanced together by one batched launch per iteration; every `snap`
@@ -69,11 +70,11 @@ class _Conv(torch.autograd.Function):
     def backward(ctx, dy):
         x, w16, y = ctx.saved_tensors
         dy = dy.to(torch.bfloat16)
+        dy = dy.contiguous()
         if ctx.relu:
             # one masking pass: measured faster here than the kernels'
             # mask-on-load (relu_y), which adds a load per operand fragment
-            dy = torch.where(y > 0, dy, torch.zeros((), dtype=dy.dtype, device=dy.device))
-        dy = dy.contiguous()
+            dy = so.relu_mask(dy, y)
         dx = None
         if ctx.needs_input_grad[0]:
             dx = so.conv3x3_bwd_data(dy, so.flip_weights(w16))
@@ -92,6 +93,24 @@ def _pool(x):
 
 def _up(x):
     return x.repeat_interleave(2, dim=1).repeat_interleave(2, dim=2)
+
+
+class _UpCat(torch.autograd.Function):
+    """torch.cat([_up(lo), skip], dim=3) on one kernel each way, bit for
+    bit the same as that composition and its autograd."""
+
+    @staticmethod
+    def forward(ctx, lo, skip):
+        ctx.c1 = lo.shape[3]
+        return so.up_cat(lo.contiguous(), skip.contiguous())
+
+    @staticmethod
+    def backward(ctx, dout):
+        return so.up_cat_bwd(dout.to(torch.bfloat16).contiguous(), ctx.c1)
+
+
+def up_cat(lo, skip):
+    return _UpCat.apply(lo, skip)
 
 
 class UNet(torch.nn.Module):
@@ -118,9 +137,9 @@ class UNet(torch.nn.Module):
         s1 = self._c("e1b", self._c("e1a", _pool(s0)))
         s2 = self._c("e2b", self._c("e2a", _pool(s1)))
         b = self._c("e3b", self._c("e3a", _pool(s2)))
-        d = self._c("d2b", self._c("d2a", torch.cat([_up(b), s2], dim=3).contiguous()))
-        d = self._c("d1b", self._c("d1a", torch.cat([_up(d), s1], dim=3).contiguous()))
-        d = self._c("d0b", self._c("d0a", torch.cat([_up(d), s0], dim=3).contiguous()))
+        d = self._c("d2b", self._c("d2a", up_cat(b, s2)))
+        d = self._c("d1b", self._c("d1a", up_cat(d, s1)))
+        d = self._c("d0b", self._c("d0a", up_cat(d, s0)))
         return self._c("head", d, relu=False, out_f32=True)[..., :3]
 
     def n_params(self):

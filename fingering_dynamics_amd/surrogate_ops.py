@@ -21,6 +21,9 @@ EXPORTS = {
     "fds_conv3x3_bwd_data": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
     "fds_bwd_weight_scratch": (ctypes.c_int64, [_I32] * 5),
     "fds_conv3x3_bwd_weight": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_up_cat": (ctypes.c_int, [_VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_up_cat_bwd": (ctypes.c_int, [_VP, _VP, _VP] + [_I32] * 5 + [_VP]),
+    "fds_relu_mask": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _VP]),
     "fds_last_error": (ctypes.c_char_p, []),
 }
 
@@ -144,3 +147,48 @@ def conv3x3_bwd_weight(x, dy, with_bias=True, scratch=None, relu_y=None):
     _check(load().fds_conv3x3_bwd_weight(_p(x), _p(dy), _p(_mask(relu_y, dy)), _p(dw), _p(db), _p(scratch), N, H,
                                          W, Cin, Cout, _stream()))
     return dw, db
+
+
+def up_cat(lo, skip):
+    """torch.cat([up2(lo), skip], dim=3) in one kernel: lo [N, H/2, W/2, C1]
+    and skip [N, H, W, C2] bf16 NHWC -> [N, H, W, C1 + C2] bf16, up2 the
+    nearest-neighbour doubling of H and W."""
+    N, h, w, C1 = _nhwc(lo, torch.bfloat16)
+    N2, H, W, C2 = _nhwc(skip, torch.bfloat16)
+    if N2 != N or H != 2 * h or W != 2 * w:
+        raise ValueError("skip must be [N, 2H, 2W, C2] for lo [N, H, W, C1]")
+    if C1 % 8 or C2 % 8:
+        raise ValueError("C1 and C2 must be multiples of 8")
+    _same_device(lo, skip)
+    out = torch.empty((N, H, W, C1 + C2), dtype=torch.bfloat16, device=lo.device)
+    _check(load().fds_up_cat(_p(lo), _p(skip), _p(out), N, H, W, C1, C2, _stream()))
+    return out
+
+
+def up_cat_bwd(dout, c1):
+    """(dlo, dskip): the gradients of up_cat's inputs from that of its
+    output, dout [N, H, W, C1 + C2] bf16 with C1 = c1; dlo sums each 2 x 2
+    block as the gradients of two nearest-neighbour repeats do (columns,
+    then rows, each rounded to bf16)."""
+    N, H, W, C = _nhwc(dout, torch.bfloat16)
+    c1 = int(c1)
+    if H % 2 or W % 2 or c1 % 8 or c1 < 8 or C - c1 < 8 or (C - c1) % 8:
+        raise ValueError("dout must be [N, H, W, C1 + C2] with H, W even and C1, C2 positive multiples of 8")
+    _same_device(dout)
+    dlo = torch.empty((N, H // 2, W // 2, c1), dtype=torch.bfloat16, device=dout.device)
+    dskip = torch.empty((N, H, W, C - c1), dtype=torch.bfloat16, device=dout.device)
+    _check(load().fds_up_cat_bwd(_p(dout), _p(dlo), _p(dskip), N, H, W, c1, C - c1, _stream()))
+    return dlo, dskip
+
+
+def relu_mask(dy, y):
+    """torch.where(y > 0, dy, 0) for bf16 tensors of one shape (the
+    backward of a ReLU whose output is y), in one kernel."""
+    if dy.dtype != torch.bfloat16 or y.dtype != torch.bfloat16 or dy.shape != y.shape:
+        raise ValueError("dy and y must be bf16 tensors of the same shape")
+    if not dy.is_contiguous() or not y.is_contiguous() or dy.numel() % 8:
+        raise ValueError("dy and y must be contiguous with a multiple of 8 elements")
+    _same_device(dy, y)
+    out = torch.empty_like(dy)
+    _check(load().fds_relu_mask(_p(dy), _p(y), _p(out), dy.numel(), _stream()))
+    return out

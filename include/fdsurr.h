@@ -55,6 +55,26 @@ int64_t fds_bwd_weight_scratch(int32_t N, int32_t H, int32_t W, int32_t Cin, int
 int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t* relu_y, float* dw, float* dbias,
                            float* scratch, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, void* stream);
 
+/* Decoder input of the U-Net: out[n][h][w][0:C1] = lo[n][h/2][w/2][:]
+ * (nearest-neighbour upsampling, lo is N x H/2 x W/2 x C1) and
+ * out[n][h][w][C1:C1+C2] = skip[n][h][w][:]; bf16 NHWC, H and W even, C1
+ * and C2 multiples of 8.  Replaces torch.cat([up(lo), skip], dim=3). */
+int fds_up_cat(const uint16_t* lo, const uint16_t* skip, uint16_t* out, int32_t N, int32_t H, int32_t W,
+               int32_t C1, int32_t C2, void* stream);
+
+/* Gradients of fds_up_cat: dlo (N x H/2 x W/2 x C1) = the sum of dout over
+ * each 2 x 2 block of its first C1 channels, the two columns of each row
+ * summed and rounded to bf16 first, then the two rows (the order of the
+ * gradients of two nearest-neighbour repeats, along w then h); dskip = the
+ * last C2 channels of dout. */
+int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t N, int32_t H, int32_t W,
+                   int32_t C1, int32_t C2, void* stream);
+
+/* out = dy where y > 0 (bf16 values as floats: -0 and NaN are not), else 0:
+ * the backward mask of a ReLU whose bf16 output is y; n values, n % 8 == 0;
+ * out may be dy. */
+int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t n, void* stream);
+
 /* Message of the last failed call on this thread. */
 const char* fds_last_error(void);
 

@@ -119,6 +119,7 @@ def test_surrogate_library_rejects_bad_arguments_before_any_launch():
     from fingering_dynamics_amd import surrogate_ops
     lib = surrogate_ops.load()
     err = lambda: lib.fds_last_error().decode()
+    bogus = ctypes.c_void_p(4096)
     cases = [
         (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 0, 8, 8, 8, 8, 1, None), "bad shape"),
         (lambda: lib.fds_conv3x3_fwd(None, None, None, None, None, 1, 8, 8, 12, 8, 1, None), "Cin must be"),
@@ -133,6 +134,16 @@ def test_surrogate_library_rejects_bad_arguments_before_any_launch():
         (lambda: lib.fds_conv3x3_bwd_weight(None, None, None, None, None, None, 1, 8, 8, 8, 8, None),
          "NULL pointer"),
         (lambda: lib.fds_flip_weights(None, None, 8, 8, None), "bad arguments"),
+        # a non-NULL address that no call below dereferences: the shape
+        # checks reject it first
+        (lambda: lib.fds_up_cat(None, bogus, bogus, 1, 8, 8, 8, 8, None), "NULL pointer"),
+        (lambda: lib.fds_up_cat(bogus, bogus, bogus, 1, 7, 8, 8, 8, None), "even H, W"),
+        (lambda: lib.fds_up_cat(bogus, bogus, bogus, 1, 8, 8, 12, 8, None), "multiples of 8"),
+        (lambda: lib.fds_up_cat_bwd(bogus, bogus, None, 1, 8, 8, 8, 8, None), "NULL pointer"),
+        (lambda: lib.fds_up_cat_bwd(bogus, bogus, bogus, 0, 8, 8, 8, 8, None), "even H, W"),
+        (lambda: lib.fds_up_cat_bwd(bogus, bogus, bogus, 1, 8, 8, 8, 0, None), "multiples of 8"),
+        (lambda: lib.fds_relu_mask(bogus, None, bogus, 8, None), "NULL pointer"),
+        (lambda: lib.fds_relu_mask(bogus, bogus, bogus, 12, None), "multiple of 8"),
     ]
     for call, msg in cases:
         assert call() == -1

@@ -102,3 +102,60 @@ def test_relu_mask_on_load_equals_premasked_gradient():
     dw_b, db_b = so.conv3x3_bwd_weight(x, pre)
     assert torch.equal(dw_a, dw_b) and torch.equal(db_a, db_b)
     assert float((y > 0).float().mean()) < 0.9                     # the mask removes something
+
+
+@pytest.mark.parametrize("shape", [(2, 4, 8, 8, 16), (3, 16, 32, 128, 128), (1, 64, 128, 64, 32), (2, 8, 4, 32, 8)])
+def test_up_cat_is_the_torch_composition_bit_for_bit(shape):
+    """fds_up_cat / fds_up_cat_bwd against torch.cat([up2(lo), skip]) and
+    its autograd (two nearest-neighbour repeats), bit for bit."""
+    from fingering_dynamics_amd import surrogate_ops as so
+    from fingering_dynamics_amd.surrogate import _up
+    N, h, w, c1, c2 = shape
+    g = torch.Generator(device="cuda").manual_seed(sum(shape))
+    lo = torch.randn(N, h, w, c1, device="cuda", generator=g).to(torch.bfloat16).requires_grad_(True)
+    skip = torch.randn(N, 2 * h, 2 * w, c2, device="cuda", generator=g).to(torch.bfloat16).requires_grad_(True)
+    ref = torch.cat([_up(lo), skip], dim=3)
+    out = so.up_cat(lo.detach(), skip.detach())
+    assert out.shape == ref.shape and torch.equal(out, ref)
+    dout = torch.randn(ref.shape, device="cuda", generator=g).to(torch.bfloat16)
+    ref.backward(dout)
+    dlo, dskip = so.up_cat_bwd(dout, c1)
+    assert torch.equal(dskip, skip.grad)
+    assert torch.equal(dlo, lo.grad), float((dlo.float() - lo.grad.float()).abs().max())
+
+
+def test_relu_mask_is_torch_where():
+    from fingering_dynamics_amd import surrogate_ops as so
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dy = torch.randn(4, 16, 32, 24, device="cuda", generator=g).to(torch.bfloat16)
+    y = torch.randn(4, 16, 32, 24, device="cuda", generator=g).to(torch.bfloat16)
+    y.view(-1)[::7] = 0.0
+    y.view(-1)[3::11] = -0.0
+    ref = torch.where(y > 0, dy, torch.zeros((), dtype=dy.dtype, device=dy.device))
+    assert torch.equal(so.relu_mask(dy, y), ref)
+
+
+def test_unet_gradients_unchanged_by_the_plumbing_kernels(monkeypatch):
+    """A training step of the U-Net gives the same loss and gradients, bit for
+    bit, with the up-cat and ReLU-mask kernels as with the PyTorch
+    operations they replace."""
+    from fingering_dynamics_amd import surrogate as sg
+    from fingering_dynamics_amd import surrogate_ops as so
+    torch.manual_seed(2)
+    x = (torch.randn(4, 64, 128, 8) * 0.5).to(torch.bfloat16).cuda()
+    target = torch.randn(4, 64, 128, 3).cuda()
+
+    def grads():
+        model = sg.UNet(seed=7).cuda()
+        loss = ((model(x) - target) ** 2).mean()
+        loss.backward()
+        return float(loss), [p.grad.clone() for p in model.parameters()]
+
+    loss_k, g_k = grads()
+    monkeypatch.setattr(sg, "up_cat", lambda lo, skip: torch.cat([sg._up(lo), skip], dim=3).contiguous())
+    monkeypatch.setattr(so, "relu_mask",
+                        lambda dy, y: torch.where(y > 0, dy, torch.zeros((), dtype=dy.dtype, device=dy.device)))
+    loss_t, g_t = grads()
+    assert loss_k == loss_t
+    for a, b in zip(g_k, g_t):
+        assert torch.equal(a, b)
