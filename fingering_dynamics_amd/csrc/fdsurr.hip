@@ -316,10 +316,12 @@ __global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__
 // with 16-byte loads (8 channels of one pixel) and stores them TRANSPOSED
 // in LDS, [channel][pixel] with rows padded to 72 elements, so that every
 // MFMA fragment (8 consecutive pixels of one channel) is one 16-byte LDS
-// read.  Lane l of wave v stages pixels (l & 31) and (l & 31) + 32 of
-// channel group (l >> 5) + 2v: for a fixed element the 64 lanes write two
-// rows 8 apart (32 banks apart) and 32 consecutive pixels, free of bank
-// conflicts.  dy is multiplied on load by the ReLU mask of dym when given.
+// read.  Lane l of wave v stages the pixel pair 2(l & 31), 2(l & 31) + 1
+// of channel group (l >> 5) + 2v, the pair's two values of a channel in
+// one 32-bit write: for a fixed element the 64 lanes write two rows 8
+// apart and 32 consecutive pixel pairs (16-bit writes of single pixels
+// were 3-6 % slower).  dy is multiplied on load by the ReLU mask of dym
+// when given.
 // A workgroup owns one tap, 64 co x 64 ci and one slice of
 // pixels; wave v computes co rows 16v..16v+15 (four 16 x 16 tiles, two
 // 32-pixel MFMA steps per stage).  The workgroups of tap 0 and the first
@@ -353,17 +355,20 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     v4f accb = {0.f, 0.f, 0.f, 0.f};
     const bool with_bias = bpart != nullptr && tap == 0 && ib == 0;
     const u4v ones = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};  // eight bf16 1.0
-    const int pl = lane & 31, cg = ((lane >> 5) + 2 * wave) * 8;
+    // lane (q, g): the pixel pair 2q, 2q + 1 of the step and channel group
+    // g; the two pixels' values of one channel go to LDS as one 32-bit
+    // write (consecutive pixels of a channel are adjacent there)
+    const int q2 = 2 * (lane & 31), cg = ((lane >> 5) + 2 * wave) * 8;
     const bool co_in = co0 + cg < Cout, ci_in = ci0 + cg < Cin;
     for (int64_t pc = pbeg; pc < pend; pc += kStep) {
+        u4v vdy[2], vx[2];
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            const int pr = pl + 32 * half;
-            const int64_t p = pc + pr;
-            u4v vdy = {0u, 0u, 0u, 0u}, vx = {0u, 0u, 0u, 0u}, vm = {~0u, ~0u, ~0u, ~0u};
+        for (int k = 0; k < 2; ++k) {
+            const int64_t p = pc + q2 + k;
+            u4v a = {0u, 0u, 0u, 0u}, v = {0u, 0u, 0u, 0u}, vm = {~0u, ~0u, ~0u, ~0u};
             if (p < pend) {
                 if (co_in) {
-                    vdy = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+                    a = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
                     if (dym) vm = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + cg);
                 }
                 const int wc = (int)(p % W);
@@ -371,14 +376,18 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
                 const int h = (int)(t % H), n = (int)(t / H);
                 const int hh = h + dh, ww = wc + dw;
                 if (ci_in && hh >= 0 && hh < H && ww >= 0 && ww < W)
-                    vx = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+                    v = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
             }
-            if (dym) vdy = relu_keep(vdy, vm);
+            vdy[k] = dym ? relu_keep(a, vm) : a;
+            vx[k] = v;
+        }
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                sdy[cg + e][pr] = (uint16_t)(vdy[e >> 1] >> (16 * (e & 1)));
-                sx[cg + e][pr] = (uint16_t)(vx[e >> 1] >> (16 * (e & 1)));
-            }
+        for (int e = 0; e < 8; ++e) {
+            const int sh = 16 * (e & 1);
+            *reinterpret_cast<unsigned*>(&sdy[cg + e][q2]) =
+                ((vdy[0][e >> 1] >> sh) & 0xFFFFu) | (((vdy[1][e >> 1] >> sh) & 0xFFFFu) << 16);
+            *reinterpret_cast<unsigned*>(&sx[cg + e][q2]) =
+                ((vx[0][e >> 1] >> sh) & 0xFFFFu) | (((vx[1][e >> 1] >> sh) & 0xFFFFu) << 16);
         }
         __syncthreads();
 #pragma unroll
@@ -419,7 +428,9 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
 // segment it stages dy (transposed, [co][pixel]) and the three x rows the
 // taps reach, each in three copies shifted by dw = -1, 0, +1
 // ([dh][dw][ci][pixel]), so every tap's fragments are aligned 16-byte LDS
-// reads and x is read once for all taps.  Wave v computes taps v, v+4, v+8
+// reads and x is read once for all taps.  Staging writes pixel pairs as
+// 32-bit words, 32 consecutive pairs of two channel rows 8 apart per wave
+// instruction.  Wave v computes taps v, v+4, v+8
 // (< 9), 16 x 16 tiles skipped where the channels end; wave 3 of the first
 // channel block also forms the bias sums (ones operand).  Partial sums per
 // slice, added by reduce_slices.
@@ -454,35 +465,52 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
         const int64_t nh = sg / wsegs;
         const int h = (int)(nh % H);
         const int64_t n = nh / H;
-        // dy: 64 pixels x 4 groups of 8 output channels, one piece per thread
-        {
-            const int pp = threadIdx.x >> 2, g = threadIdx.x & 3;
-            u4v v = {0u, 0u, 0u, 0u}, m = {~0u, ~0u, ~0u, ~0u};
-            if (w0 + pp < W && 8 * g < Cout) {
-                const int64_t o = ((n * H + h) * W + w0 + pp) * Cout + 8 * g;
-                v = *reinterpret_cast<const u4v*>(dy + o);
-                if (dym) m = *reinterpret_cast<const u4v*>(dym + o);
-            }
-            if (dym) v = relu_keep(v, m);
+        // dy: 32 pixel pairs x 4 groups of 8 output channels on threads
+        // 0..127, lane t of group g taking pixels 2t, 2t + 1
+        if (threadIdx.x < 128) {
+            const int t = threadIdx.x & 31, g = threadIdx.x >> 5;
+            u4v v[2];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) sdy[8 * g + e][pp] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+            for (int k = 0; k < 2; ++k) {
+                u4v a = {0u, 0u, 0u, 0u}, m = {~0u, ~0u, ~0u, ~0u};
+                if (w0 + 2 * t + k < W && 8 * g < Cout) {
+                    const int64_t o = ((n * H + h) * W + w0 + 2 * t + k) * Cout + 8 * g;
+                    a = *reinterpret_cast<const u4v*>(dy + o);
+                    if (dym) m = *reinterpret_cast<const u4v*>(dym + o);
+                }
+                v[k] = dym ? relu_keep(a, m) : a;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int sh = 16 * (e & 1);
+                *reinterpret_cast<unsigned*>(&sdy[8 * g + e][2 * t]) =
+                    ((v[0][e >> 1] >> sh) & 0xFFFFu) | (((v[1][e >> 1] >> sh) & 0xFFFFu) << 16);
+            }
         }
-        // x: 3 rows x 66 columns (w0-1 .. w0+64) x 4 groups of 8 input
-        // channels; column c lands at pixel c - dw of copy dw
-        for (int i = threadIdx.x; i < 3 * 66 * 4; i += 256) {
-            const int g = i & 3, c = (i >> 2) % 66 - 1, r = (i >> 2) / 66;
-            const int hh = h + r - 1, ww = w0 + c;
-            u4v v = {0u, 0u, 0u, 0u};
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W && ci0 + 8 * g < Cin)
-                v = *reinterpret_cast<const u4v*>(x + ((n * H + hh) * W + ww) * Cin + ci0 + 8 * g);
+        // x: 3 rows x 32 pixel pairs x 4 groups of 8 input channels; the
+        // item (r, g, t) loads columns w0 + 2t - 1 .. w0 + 2t + 2 and writes
+        // pixels 2t, 2t + 1 of the three copies (copy dw holds column
+        // pp + dw at pixel pp), each channel's pair as one 32-bit write
+        for (int i = threadIdx.x; i < 3 * 4 * 32; i += 256) {
+            const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
+            const int hh = h + r - 1;
+            const bool rowok = hh >= 0 && hh < H && ci0 + 8 * g < Cin;
+            u4v v[4];
 #pragma unroll
-            for (int dw = -1; dw <= 1; ++dw) {
-                const int pp = c - dw;
-                if (pp < 0 || pp >= 64) continue;
-#pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    sx[3 * r + dw + 1][8 * g + e][pp] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+            for (int k = 0; k < 4; ++k) {
+                const int ww = w0 + 2 * t - 1 + k;
+                v[k] = u4v{0u, 0u, 0u, 0u};
+                if (rowok && ww >= 0 && ww < W)
+                    v[k] = *reinterpret_cast<const u4v*>(x + ((n * H + hh) * W + ww) * Cin + ci0 + 8 * g);
             }
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int sh = 16 * (e & 1);
+                    *reinterpret_cast<unsigned*>(&sx[3 * r + d][8 * g + e][2 * t]) =
+                        ((v[d][e >> 1] >> sh) & 0xFFFFu) | (((v[d + 1][e >> 1] >> sh) & 0xFFFFu) << 16);
+                }
         }
         __syncthreads();
 #pragma unroll
