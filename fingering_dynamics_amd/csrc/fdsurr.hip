@@ -66,6 +66,17 @@ __device__ __forceinline__ u4v relu_keep(u4v v, u4v m) {
 // takes 32 k per step (one tap and 8 channels per lane, a 16-byte load);
 // k beyond 9*Cin and channels beyond Cout load zeros.  With MASK, x is
 // multiplied on load by the ReLU mask of xm (the input gradient pass).
+// Workgroups of a 1-D grid are dispatched to the 8 XCDs round robin
+// (workgroup i runs on XCD i mod 8), and each XCD has its own L2.  The
+// logical index returned gives every XCD a contiguous band of logical
+// workgroups, so workgroups adjacent in logical order, which read the same
+// or overlapping data (the channel blocks of one tile, the taps of one
+// pixel slice, neighbouring tiles' halo rows), share an L2.
+__device__ __forceinline__ int xcd_band() {
+    const int n = (int)gridDim.x, q = n >> 3, rem = n & 7, xcd = (int)blockIdx.x & 7;
+    return xcd * q + min(xcd, rem) + ((int)blockIdx.x >> 3);
+}
+
 template <int MP, int NT, bool MASK>
 __global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ xm,
                                                 const uint16_t* __restrict__ w, const float* __restrict__ bias,
@@ -73,8 +84,12 @@ __global__ __launch_bounds__(256) void conv_fwd(const uint16_t* __restrict__ x, 
                                                 int W, int Cin, int Cout, int relu) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t P = (int64_t)N * H * W;
-    const int64_t pw = ((int64_t)blockIdx.x * 4 + wave) * (16 * MP);
-    const int co0 = blockIdx.y * (16 * NT);
+    // logical workgroup = pixel block x channel block, channel blocks adjacent
+    const int ncb = (Cout + 16 * NT - 1) / (16 * NT);
+    const int L = xcd_band();
+    const int pb = L / ncb;
+    const int64_t pw = ((int64_t)pb * 4 + wave) * (16 * MP);
+    const int co0 = (L - pb * ncb) * (16 * NT);
     const int K = 9 * Cin;
     int hq[MP], wq[MP];
     int64_t img[MP];
@@ -181,13 +196,17 @@ __global__ __launch_bounds__(256) void conv_fwd_lds(const uint16_t* __restrict__
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave % WR, wn = wave / WR;
     const int wblocks = (W + 31) / 32, hblocks = (H + BR - 1) / BR;
-    int b = blockIdx.x;
+    // logical workgroup = tile x channel block, channel blocks adjacent
+    const int ncb = (Cout + WN * WC * 16 - 1) / (WN * WC * 16);
+    const int L = xcd_band();
+    int b = L / ncb;
+    const int cb = L - b * ncb;
     const int bw = b % wblocks;
     b /= wblocks;
     const int bh = b % hblocks;
     const int n = b / hblocks;
     const int h0 = bh * BR, w0 = bw * 32;
-    const int co0 = blockIdx.y * (WN * WC * 16) + wn * (WC * 16);
+    const int co0 = cb * (WN * WC * 16) + wn * (WC * 16);
     const int K = 9 * Cin;
     const int64_t img = (int64_t)n * H;
 
@@ -340,14 +359,19 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     __shared__ __align__(16) uint16_t sx[64][kLds];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ncb = (Cout + 63) / 64;
-    int gy = blockIdx.y;
+    // logical workgroup = slice x (ci block, tap, co block), the 9 taps and
+    // channel blocks of one slice adjacent (they read the same dy and x)
+    const int G = ncb * 9 * ((Cin + 63) / 64);
+    const int L = xcd_band();
+    const int slice = L / G;
+    int gy = L - slice * G;
     const int cb = gy % ncb;
     gy /= ncb;
     const int tap = gy % 9;
     const int ib = gy / 9;
     const int co0 = cb * 64, ci0 = ib * 64, dh = tap / 3 - 1, dw = tap % 3 - 1;
     const int64_t P = (int64_t)N * H * W;
-    const int64_t pbeg = (int64_t)blockIdx.x * chunk;
+    const int64_t pbeg = (int64_t)slice * chunk;
     const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
     v4f acc[4];
 #pragma unroll
@@ -360,35 +384,64 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     // write (consecutive pixels of a channel are adjacent there)
     const int q2 = 2 * (lane & 31), cg = ((lane >> 5) + 2 * wave) * 8;
     const bool co_in = co0 + cg < Cout, ci_in = ci0 + cg < Cin;
-    for (int64_t pc = pbeg; pc < pend; pc += kStep) {
-        u4v vdy[2], vx[2];
+    // the next step's operands are loaded once this step's are in LDS, so
+    // the loads run during the barrier and the MFMAs
+    u4v ldy[2], lm[2], lx[2];
+    // (n, h, wc) of the lane's first pixel of the next load, divided out
+    // once and then advanced by kStep pixels per step (64-bit divisions per
+    // pixel and step cost more than the step's MFMAs)
+    int pn, ph, pwc;
+    {
+        const int64_t p0 = pbeg + q2, t = p0 / W;
+        pwc = (int)(p0 - t * W);
+        pn = (int)(t / H);
+        ph = (int)(t - (int64_t)pn * H);
+    }
+    auto advance = [&](int& n, int& h, int& wc, int d) {
+        wc += d;
+        while (wc >= W) {
+            wc -= W;
+            if (++h == H) {
+                h = 0;
+                ++n;
+            }
+        }
+    };
+    auto load = [&](int64_t pc) {
+        int n = pn, h = ph, wc = pwc;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int64_t p = pc + q2 + k;
-            u4v a = {0u, 0u, 0u, 0u}, v = {0u, 0u, 0u, 0u}, vm = {~0u, ~0u, ~0u, ~0u};
+            if (k) advance(n, h, wc, 1);
+            ldy[k] = u4v{0u, 0u, 0u, 0u};
+            lx[k] = u4v{0u, 0u, 0u, 0u};
+            lm[k] = u4v{~0u, ~0u, ~0u, ~0u};
             if (p < pend) {
                 if (co_in) {
-                    a = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
-                    if (dym) vm = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + cg);
+                    ldy[k] = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
+                    if (dym) lm[k] = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + cg);
                 }
-                const int wc = (int)(p % W);
-                const int64_t t = p / W;
-                const int h = (int)(t % H), n = (int)(t / H);
                 const int hh = h + dh, ww = wc + dw;
                 if (ci_in && hh >= 0 && hh < H && ww >= 0 && ww < W)
-                    v = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+                    lx[k] = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
             }
-            vdy[k] = dym ? relu_keep(a, vm) : a;
-            vx[k] = v;
         }
+        advance(pn, ph, pwc, kStep);
+    };
+    if (pbeg < pend) load(pbeg);
+    for (int64_t pc = pbeg; pc < pend; pc += kStep) {
+        u4v vdy[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) vdy[k] = dym ? relu_keep(ldy[k], lm[k]) : ldy[k];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int sh = 16 * (e & 1);
             *reinterpret_cast<unsigned*>(&sdy[cg + e][q2]) =
                 ((vdy[0][e >> 1] >> sh) & 0xFFFFu) | (((vdy[1][e >> 1] >> sh) & 0xFFFFu) << 16);
             *reinterpret_cast<unsigned*>(&sx[cg + e][q2]) =
-                ((vx[0][e >> 1] >> sh) & 0xFFFFu) | (((vx[1][e >> 1] >> sh) & 0xFFFFu) << 16);
+                ((lx[0][e >> 1] >> sh) & 0xFFFFu) | (((lx[1][e >> 1] >> sh) & 0xFFFFu) << 16);
         }
+        if (pc + kStep < pend) load(pc + kStep);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
@@ -410,14 +463,14 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
-            if (co < Cout && ci < Cin) part[((int64_t)blockIdx.x * Cout + co) * K + tap * Cin + ci] = acc[t][r];
+            if (co < Cout && ci < Cin) part[((int64_t)slice * Cout + co) * K + tap * Cin + ci] = acc[t][r];
         }
     }
     if (with_bias && (lane & 15) == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
-            if (co < Cout) bpart[(int64_t)blockIdx.x * Cout + co] = accb[r];
+            if (co < Cout) bpart[(int64_t)slice * Cout + co] = accb[r];
         }
     }
 }
@@ -444,13 +497,17 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
     __shared__ __align__(16) uint16_t sdy[kSmallCo][kLds];
     __shared__ __align__(16) uint16_t sx[9][kSmallCi][kLds];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ci0 = blockIdx.y * kSmallCi;
+    // logical workgroup = slice x ci block, the ci blocks of one slice adjacent
+    const int nb = (Cin + kSmallCi - 1) / kSmallCi;
+    const int L = xcd_band();
+    const int slice = L / nb, ib = L - slice * nb;
+    const int ci0 = ib * kSmallCi;
     const int wsegs = (W + 63) / 64;
     const int64_t nseg = (int64_t)N * H * wsegs;
-    const int64_t sbeg = (int64_t)blockIdx.x * segs_per_slice;
+    const int64_t sbeg = (int64_t)slice * segs_per_slice;
     const int64_t send = std::min<int64_t>(nseg, sbeg + segs_per_slice);
     const bool co2 = Cout > 16, ci2 = ci0 + 16 < Cin;
-    const bool with_bias = bpart != nullptr && blockIdx.y == 0 && wave == 3;
+    const bool with_bias = bpart != nullptr && ib == 0 && wave == 3;
     const u4v ones = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};
     v4f acc[3][2][2];
 #pragma unroll
@@ -460,58 +517,79 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
 #pragma unroll
             for (int b = 0; b < 2; ++b) acc[j][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
     v4f accb[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
-    for (int64_t sg = sbeg; sg < send; ++sg) {
+    // The next segment's operands are loaded once this segment's are in LDS,
+    // so the loads run during the barrier and the MFMAs.  dy: 32 pixel pairs x 4 groups of 8 output
+    // channels on threads 0..127, lane t of group g taking pixels 2t, 2t + 1.
+    // x: 3 rows x 32 pixel pairs x 4 groups of 8 input channels, items
+    // i = (r, g, t) = thread and thread + 256 (threads 0..127); item i loads
+    // columns w0 + 2t - 1 .. w0 + 2t + 2 and writes pixels 2t, 2t + 1 of the
+    // three copies (copy dw holds column pp + dw at pixel pp), each
+    // channel's pair as one 32-bit write.
+    const bool stage_dy = threadIdx.x < 128, two_items = threadIdx.x < 128;
+    u4v ldy[2], lm[2], lx[2][4];
+    auto load = [&](int64_t sg) {
         const int w0 = (int)(sg % wsegs) * 64;
         const int64_t nh = sg / wsegs;
         const int h = (int)(nh % H);
         const int64_t n = nh / H;
-        // dy: 32 pixel pairs x 4 groups of 8 output channels on threads
-        // 0..127, lane t of group g taking pixels 2t, 2t + 1
-        if (threadIdx.x < 128) {
+        if (stage_dy) {
             const int t = threadIdx.x & 31, g = threadIdx.x >> 5;
-            u4v v[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                u4v a = {0u, 0u, 0u, 0u}, m = {~0u, ~0u, ~0u, ~0u};
+                ldy[k] = u4v{0u, 0u, 0u, 0u};
+                lm[k] = u4v{~0u, ~0u, ~0u, ~0u};
                 if (w0 + 2 * t + k < W && 8 * g < Cout) {
                     const int64_t o = ((n * H + h) * W + w0 + 2 * t + k) * Cout + 8 * g;
-                    a = *reinterpret_cast<const u4v*>(dy + o);
-                    if (dym) m = *reinterpret_cast<const u4v*>(dym + o);
+                    ldy[k] = *reinterpret_cast<const u4v*>(dy + o);
+                    if (dym) lm[k] = *reinterpret_cast<const u4v*>(dym + o);
                 }
-                v[k] = dym ? relu_keep(a, m) : a;
             }
+        }
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            if (it == 1 && !two_items) break;
+            const int i = threadIdx.x + 256 * it;
+            const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
+            const int hh = h + r - 1;
+            const bool rowok = hh >= 0 && hh < H && ci0 + 8 * g < Cin;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int ww = w0 + 2 * t - 1 + k;
+                lx[it][k] = u4v{0u, 0u, 0u, 0u};
+                if (rowok && ww >= 0 && ww < W)
+                    lx[it][k] = *reinterpret_cast<const u4v*>(x + ((n * H + hh) * W + ww) * Cin + ci0 + 8 * g);
+            }
+        }
+    };
+    if (sbeg < send) load(sbeg);
+    for (int64_t sg = sbeg; sg < send; ++sg) {
+        u4v vdy[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) vdy[k] = dym ? relu_keep(ldy[k], lm[k]) : ldy[k];
+        if (stage_dy) {
+            const int t = threadIdx.x & 31, g = threadIdx.x >> 5;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int sh = 16 * (e & 1);
                 *reinterpret_cast<unsigned*>(&sdy[8 * g + e][2 * t]) =
-                    ((v[0][e >> 1] >> sh) & 0xFFFFu) | (((v[1][e >> 1] >> sh) & 0xFFFFu) << 16);
+                    ((vdy[0][e >> 1] >> sh) & 0xFFFFu) | (((vdy[1][e >> 1] >> sh) & 0xFFFFu) << 16);
             }
         }
-        // x: 3 rows x 32 pixel pairs x 4 groups of 8 input channels; the
-        // item (r, g, t) loads columns w0 + 2t - 1 .. w0 + 2t + 2 and writes
-        // pixels 2t, 2t + 1 of the three copies (copy dw holds column
-        // pp + dw at pixel pp), each channel's pair as one 32-bit write
-        for (int i = threadIdx.x; i < 3 * 4 * 32; i += 256) {
-            const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
-            const int hh = h + r - 1;
-            const bool rowok = hh >= 0 && hh < H && ci0 + 8 * g < Cin;
-            u4v v[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int ww = w0 + 2 * t - 1 + k;
-                v[k] = u4v{0u, 0u, 0u, 0u};
-                if (rowok && ww >= 0 && ww < W)
-                    v[k] = *reinterpret_cast<const u4v*>(x + ((n * H + hh) * W + ww) * Cin + ci0 + 8 * g);
-            }
+        for (int it = 0; it < 2; ++it) {
+            if (it == 1 && !two_items) break;
+            const int i = threadIdx.x + 256 * it;
+            const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int sh = 16 * (e & 1);
                     *reinterpret_cast<unsigned*>(&sx[3 * r + d][8 * g + e][2 * t]) =
-                        ((v[d][e >> 1] >> sh) & 0xFFFFu) | (((v[d + 1][e >> 1] >> sh) & 0xFFFFu) << 16);
+                        ((lx[it][d][e >> 1] >> sh) & 0xFFFFu) | (((lx[it][d + 1][e >> 1] >> sh) & 0xFFFFu) << 16);
                 }
         }
+        if (sg + 1 < send) load(sg + 1);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
@@ -552,7 +630,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
                 for (int r = 0; r < 4; ++r) {
                     const int co = 16 * a + 4 * (lane >> 4) + r;
                     if (co < Cout && ci < Cin && ci < ci0 + kSmallCi)
-                        part[((int64_t)blockIdx.x * Cout + co) * K + tap * Cin + ci] = acc[j][a][b][r];
+                        part[((int64_t)slice * Cout + co) * K + tap * Cin + ci] = acc[j][a][b][r];
                 }
             }
     }
@@ -562,7 +640,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int co = 16 * a + 4 * (lane >> 4) + r;
-                if (co < Cout) bpart[(int64_t)blockIdx.x * Cout + co] = accb[a][r];
+                if (co < Cout) bpart[(int64_t)slice * Cout + co] = accb[a][r];
             }
     }
 }
@@ -721,15 +799,15 @@ void launch_fwd(const uint16_t* x, const uint16_t* xm, const uint16_t* w, const 
         // per-layer times favour the direct kernel below when Cin is 32
         const unsigned wb = (unsigned)((W + 31) / 32);
         if (Cout < 64) {
-            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N, (unsigned)((Cout + 31) / 32));
+            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N * (unsigned)((Cout + 31) / 32));
             hipLaunchKernelGGL((conv_fwd_lds<4, 2, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
                                Cin, Cout, relu);
         } else if (Cout % 128 == 0) {
-            const dim3 grid(wb * (unsigned)((H + 3) / 4) * (unsigned)N, (unsigned)((Cout + 127) / 128));
+            const dim3 grid(wb * (unsigned)((H + 3) / 4) * (unsigned)N * (unsigned)((Cout + 127) / 128));
             hipLaunchKernelGGL((conv_fwd_lds<2, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
                                Cin, Cout, relu);
         } else {
-            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N, (unsigned)((Cout + 63) / 64));
+            const dim3 grid(wb * (unsigned)((H + 7) / 8) * (unsigned)N * (unsigned)((Cout + 63) / 64));
             hipLaunchKernelGGL((conv_fwd_lds<4, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W,
                                Cin, Cout, relu);
         }
@@ -737,15 +815,15 @@ void launch_fwd(const uint16_t* x, const uint16_t* xm, const uint16_t* w, const 
     }
     // wave tiles: 64 px x 64 co, 128 px x 32 co, or 128 px x 16 co
     if (Cout >= 64) {
-        const dim3 grid((unsigned)((P + 255) / 256), (unsigned)((Cout + 63) / 64));
+        const dim3 grid((unsigned)((P + 255) / 256) * (unsigned)((Cout + 63) / 64));
         hipLaunchKernelGGL((conv_fwd<4, 4, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
                            Cout, relu);
     } else if (Cout >= 32) {
-        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 31) / 32));
+        const dim3 grid((unsigned)((P + 511) / 512) * (unsigned)((Cout + 31) / 32));
         hipLaunchKernelGGL((conv_fwd<8, 2, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
                            Cout, relu);
     } else {
-        const dim3 grid((unsigned)((P + 511) / 512), (unsigned)((Cout + 15) / 16));
+        const dim3 grid((unsigned)((P + 511) / 512) * (unsigned)((Cout + 15) / 16));
         hipLaunchKernelGGL((conv_fwd<8, 1, MASK>), grid, dim3(256), 0, st, x, xm, w, bias, y32, y16, N, H, W, Cin,
                            Cout, relu);
     }
@@ -810,11 +888,11 @@ int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t
     const int64_t nw = (int64_t)Cout * 9 * Cin;
     float* bp = dbias ? scratch + (int64_t)S * nw : nullptr;
     if (pl.small) {
-        const dim3 grid((unsigned)S, (unsigned)((Cin + kSmallCi - 1) / kSmallCi));
+        const dim3 grid((unsigned)S * (unsigned)((Cin + kSmallCi - 1) / kSmallCi));
         hipLaunchKernelGGL(conv_bwd_w_small, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout,
                            pl.chunk);
     } else {
-        const dim3 grid((unsigned)S, (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
+        const dim3 grid((unsigned)S * (unsigned)(((Cout + 63) / 64) * 9 * ((Cin + 63) / 64)));
         hipLaunchKernelGGL(conv_bwd_w, grid, dim3(256), 0, st, x, dy, relu_y, scratch, bp, N, H, W, Cin, Cout,
                            pl.chunk);
     }
