@@ -341,12 +341,15 @@ __global__ __launch_bounds__(256) void flip_weights(const uint16_t* __restrict__
 // apart and 32 consecutive pixel pairs (16-bit writes of single pixels
 // were 3-6 % slower).  dy is multiplied on load by the ReLU mask of dym
 // when given.
-// A workgroup owns one tap, 64 co x 64 ci and one slice of
-// pixels; wave v computes co rows 16v..16v+15 (four 16 x 16 tiles, two
-// 32-pixel MFMA steps per stage).  The workgroups of tap 0 and the first
-// ci block also form the bias gradient's slice sums, sum_p dy[p][co], with
-// one more MFMA per step against a ones operand (every column of the tile
-// then holds the row sum).  Slices write partial sums that reduce_slices
+// A workgroup owns one tap, 64 co x 64 ci and one slice of pixels; wave
+// (wrow, wcol) = (v >> 1, v & 1) computes co 32 wrow .. 32 wrow + 31 x ci
+// 32 wcol .. 32 wcol + 31 (2 x 2 tiles of 16 x 16, two 32-pixel MFMA steps per stage: four
+// fragment reads per four MFMAs).  LDS holds two stages: the next step is
+// written into the other buffer while this one's MFMAs run, one barrier per
+// step.  The workgroups of tap 0 and the first ci block also form the bias
+// gradient's slice sums, sum_p dy[p][co], on the waves with wcol = 0, with
+// one more MFMA per co tile against a ones operand (every column of the
+// tile then holds the row sum).  Slices write partial sums that reduce_slices
 // adds in a fixed order.
 constexpr int kStep = 64;
 constexpr int kLds = kStep + 8;
@@ -355,9 +358,10 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
                                                   const uint16_t* __restrict__ dym, float* __restrict__ part,
                                                   float* __restrict__ bpart, int N, int H, int W, int Cin, int Cout,
                                                   int64_t chunk) {
-    __shared__ __align__(16) uint16_t sdy[64][kLds];
-    __shared__ __align__(16) uint16_t sx[64][kLds];
+    __shared__ __align__(16) uint16_t sdy[2][64][kLds];
+    __shared__ __align__(16) uint16_t sx[2][64][kLds];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wrow = wave >> 1, wcol = wave & 1;
     const int ncb = (Cout + 63) / 64;
     // logical workgroup = slice x (ci block, tap, co block), the 9 taps and
     // channel blocks of one slice adjacent (they read the same dy and x)
@@ -373,26 +377,33 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
     const int64_t P = (int64_t)N * H * W;
     const int64_t pbeg = (int64_t)slice * chunk;
     const int64_t pend = std::min<int64_t>(P, pbeg + chunk);
-    v4f acc[4];
+    v4f acc[2][2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-    v4f accb = {0.f, 0.f, 0.f, 0.f};
-    const bool with_bias = bpart != nullptr && tap == 0 && ib == 0;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    v4f accb[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+    const bool with_bias = bpart != nullptr && tap == 0 && ib == 0 && wcol == 0;
     const u4v ones = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};  // eight bf16 1.0
-    // lane (q, g): the pixel pair 2q, 2q + 1 of the step and channel group
-    // g; the two pixels' values of one channel go to LDS as one 32-bit
-    // write (consecutive pixels of a channel are adjacent there)
-    const int q2 = 2 * (lane & 31), cg = ((lane >> 5) + 2 * wave) * 8;
-    const bool co_in = co0 + cg < Cout, ci_in = ci0 + cg < Cin;
+    // Thread t loads channels g8 .. g8 + 7 (g8 = 8 (t & 7)) of pixels
+    // pl = t >> 3 and pl + 32 of the step: eight consecutive lanes read one
+    // pixel's 64 channels, 128 contiguous bytes.  Lanes t and t ^ 8 hold
+    // adjacent pixels 2m, 2m + 1 (m = pl >> 1); one DPP row rotation by 8
+    // swaps half of each one's channels, after which the even lane holds both
+    // pixels of channels g8 .. g8 + 3 and the odd lane those of g8 + 4 ..
+    // g8 + 7, written as 32-bit pixel pairs.
+    const int g8 = (threadIdx.x & 7) * 8, pl = threadIdx.x >> 3;
+    const bool odd = pl & 1;
+    const bool co_in = co0 + g8 < Cout, ci_in = ci0 + g8 < Cin;
     // the next step's operands are loaded once this step's are in LDS, so
     // the loads run during the barrier and the MFMAs
     u4v ldy[2], lm[2], lx[2];
-    // (n, h, wc) of the lane's first pixel of the next load, divided out
+    // (n, h, wc) of the thread's first pixel of the next load, divided out
     // once and then advanced by kStep pixels per step (64-bit divisions per
     // pixel and step cost more than the step's MFMAs)
     int pn, ph, pwc;
     {
-        const int64_t p0 = pbeg + q2, t = p0 / W;
+        const int64_t p0 = pbeg + pl, t = p0 / W;
         pwc = (int)(p0 - t * W);
         pn = (int)(t / H);
         ph = (int)(t - (int64_t)pn * H);
@@ -411,67 +422,107 @@ __global__ __launch_bounds__(256) void conv_bwd_w(const uint16_t* __restrict__ x
         int n = pn, h = ph, wc = pwc;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const int64_t p = pc + q2 + k;
-            if (k) advance(n, h, wc, 1);
+            const int64_t p = pc + pl + 32 * k;
+            if (k) advance(n, h, wc, 32);
             ldy[k] = u4v{0u, 0u, 0u, 0u};
             lx[k] = u4v{0u, 0u, 0u, 0u};
             lm[k] = u4v{~0u, ~0u, ~0u, ~0u};
             if (p < pend) {
                 if (co_in) {
-                    ldy[k] = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + cg);
-                    if (dym) lm[k] = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + cg);
+                    ldy[k] = *reinterpret_cast<const u4v*>(dy + p * Cout + co0 + g8);
+                    if (dym) lm[k] = *reinterpret_cast<const u4v*>(dym + p * Cout + co0 + g8);
                 }
                 const int hh = h + dh, ww = wc + dw;
                 if (ci_in && hh >= 0 && hh < H && ww >= 0 && ww < W)
-                    lx[k] = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + cg);
+                    lx[k] = *reinterpret_cast<const u4v*>(x + (((int64_t)n * H + hh) * W + ww) * Cin + ci0 + g8);
             }
         }
         advance(pn, ph, pwc, kStep);
     };
-    if (pbeg < pend) load(pbeg);
-    for (int64_t pc = pbeg; pc < pend; pc += kStep) {
-        u4v vdy[2];
+    // LDS layout [channel R][pixel], the 16-byte chunk c (8 pixels) of row R
+    // stored at chunk c ^ (R >> 3): the pair writes of the eight channel
+    // groups and the fragment reads of 16 consecutive rows spread over the
+    // banks
+    auto pair_col = [](int R, int m) { return ((((m >> 2) ^ (R >> 3)) << 2) | (m & 3)) * 2; };
+    auto swap8 = [](unsigned v) { return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false); };
+    auto put = [&](uint16_t (*sb)[kLds], const u4v& v, int k) {
+        const unsigned r0 = swap8(odd ? v[0] : v[2]), r1 = swap8(odd ? v[1] : v[3]);
+        const unsigned o0 = odd ? v[2] : v[0], o1 = odd ? v[3] : v[1];
+        const unsigned e0 = odd ? r0 : o0, e1 = odd ? r1 : o1;  // pixel 2m
+        const unsigned d0 = odd ? o0 : r0, d1 = odd ? o1 : r1;  // pixel 2m + 1
+        const int m = (pl >> 1) + 16 * k;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) vdy[k] = dym ? relu_keep(ldy[k], lm[k]) : ldy[k];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int sh = 16 * (e & 1);
-            *reinterpret_cast<unsigned*>(&sdy[cg + e][q2]) =
-                ((vdy[0][e >> 1] >> sh) & 0xFFFFu) | (((vdy[1][e >> 1] >> sh) & 0xFFFFu) << 16);
-            *reinterpret_cast<unsigned*>(&sx[cg + e][q2]) =
-                ((lx[0][e >> 1] >> sh) & 0xFFFFu) | (((lx[1][e >> 1] >> sh) & 0xFFFFu) << 16);
+        for (int j = 0; j < 4; ++j) {
+            const int R = g8 + 4 * odd + j, sh = 16 * (j & 1);
+            const unsigned ev = j < 2 ? e0 : e1, ov = j < 2 ? d0 : d1;
+            *reinterpret_cast<unsigned*>(&sb[R][pair_col(R, m)]) = ((ev >> sh) & 0xFFFFu) | (((ov >> sh) & 0xFFFFu) << 16);
         }
-        if (pc + kStep < pend) load(pc + kStep);
-        __syncthreads();
+    };
+    // LDS stage from the load registers (the ReLU mask applied here)
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            put(sdy[buf], dym ? relu_keep(ldy[k], lm[k]) : ldy[k], k);
+            put(sx[buf], lx[k], k);
+        }
+    };
+    if (pbeg < pend) {
+        load(pbeg);
+        stage(0);
+        if (pbeg + kStep < pend) load(pbeg + kStep);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int64_t pc = pbeg; pc < pend; pc += kStep) {
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-            const int kp = 32 * kk + 8 * (lane >> 4);
-            const v8bf a = bf8(*reinterpret_cast<const u4v*>(&sdy[16 * wave + (lane & 15)][kp]));
+            const int kc = 4 * kk + (lane >> 4);  // 8-pixel chunk of the fragment
+            v8bf a[2], bv[2];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const v8bf b = bf8(*reinterpret_cast<const u4v*>(&sx[16 * t + (lane & 15)][kp]));
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            for (int i = 0; i < 2; ++i) {
+                const int ra = 32 * wrow + 16 * i + (lane & 15), rb = 32 * wcol + 16 * i + (lane & 15);
+                a[i] = bf8(*reinterpret_cast<const u4v*>(&sdy[buf][ra][8 * (kc ^ (ra >> 3))]));
+                bv[i] = bf8(*reinterpret_cast<const u4v*>(&sx[buf][rb][8 * (kc ^ (rb >> 3))]));
             }
-            if (with_bias) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf8(ones), accb, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv[j], acc[i][j], 0, 0, 0);
+            if (with_bias) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    accb[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bf8(ones), accb[i], 0, 0, 0);
+            }
+        }
+        // the other buffer was last read before the previous barrier
+        if (pc + kStep < pend) {
+            stage(buf ^ 1);
+            if (pc + 2 * kStep < pend) load(pc + 2 * kStep);
         }
         __syncthreads();
+        buf ^= 1;
     }
     const int64_t K = 9 * (int64_t)Cin;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int ci = ci0 + 16 * t + (lane & 15);
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
-            if (co < Cout && ci < Cin) part[((int64_t)slice * Cout + co) * K + tap * Cin + ci] = acc[t][r];
+        for (int j = 0; j < 2; ++j) {
+            const int ci = ci0 + 32 * wcol + 16 * j + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + 32 * wrow + 16 * i + 4 * (lane >> 4) + r;
+                if (co < Cout && ci < Cin) part[((int64_t)slice * Cout + co) * K + tap * Cin + ci] = acc[i][j][r];
+            }
         }
-    }
     if (with_bias && (lane & 15) == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
-            if (co < Cout) bpart[(int64_t)slice * Cout + co] = accb[r];
-        }
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + 32 * wrow + 16 * i + 4 * (lane >> 4) + r;
+                if (co < Cout) bpart[(int64_t)slice * Cout + co] = accb[i][r];
+            }
     }
 }
 
