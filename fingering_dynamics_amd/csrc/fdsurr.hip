@@ -696,21 +696,26 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
     }
 }
 
-// out[i] = sum over s of part[s][i]: eight interleaved partial sums (slices
-// s = j mod 8) keep eight loads in flight per thread, then a fixed tree;
-// the order is the same on every call.
+// out[i] = sum over s of part[s][i]: eight interleaved partial sums a_j
+// (slices s = j mod 8 in ascending order, from 0), then a fixed tree of the
+// eight; the order is the same on every call.  A workgroup takes 32
+// consecutive outputs, thread t the partial sum j = t >> 5 of output t & 31
+// (eight times the threads of one per output: the sums are long, the
+// outputs few), and the tree runs from LDS.
 __global__ __launch_bounds__(256) void reduce_slices(const float* __restrict__ part, float* __restrict__ out,
                                                      int64_t n, int S) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    int k = 0;
-    for (; k + 8 <= S; k += 8) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] += part[(int64_t)(k + j) * n + i];
+    __shared__ float sa[8][33];
+    const int j = threadIdx.x >> 5, ii = threadIdx.x & 31;
+    const int64_t i = (int64_t)blockIdx.x * 32 + ii;
+    float a = 0.f;
+    if (i < n) {
+#pragma unroll 4
+        for (int s = j; s < S; s += 8) a += part[(int64_t)s * n + i];
     }
-    for (int j = 0; k < S; ++k, ++j) a[j] += part[(int64_t)k * n + i];
-    out[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    sa[j][ii] = a;
+    __syncthreads();
+    if (j == 0 && i < n)
+        out[i] = ((sa[0][ii] + sa[1][ii]) + (sa[2][ii] + sa[3][ii])) + ((sa[4][ii] + sa[5][ii]) + (sa[6][ii] + sa[7][ii]));
 }
 
 // ------------------------------------------------------ U-Net plumbing
@@ -948,10 +953,10 @@ int fds_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dy, const uint16_t
                            pl.chunk);
     }
     FDS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, scratch, dw, nw, S);
+    hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((nw + 31) / 32)), dim3(256), 0, st, scratch, dw, nw, S);
     FDS_HIP(hipGetLastError());
     if (dbias) {
-        hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((Cout + 255) / 256)), dim3(256), 0, st, bp, dbias,
+        hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((Cout + 31) / 32)), dim3(256), 0, st, bp, dbias,
                            (int64_t)Cout, S);
         FDS_HIP(hipGetLastError());
     }
