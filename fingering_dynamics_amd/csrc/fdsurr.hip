@@ -569,21 +569,22 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
             for (int b = 0; b < 2; ++b) acc[j][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
     v4f accb[2] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
     // The next segment's operands are loaded once this segment's are in LDS,
-    // so the loads run during the barrier and the MFMAs.  dy: 32 pixel pairs x 4 groups of 8 output
-    // channels on threads 0..127, lane t of group g taking pixels 2t, 2t + 1.
+    // so the loads run during the barrier and the MFMAs.
+    // dy: 32 pixel pairs x 4 groups of 8 output channels on threads 0..127,
+    // lane t of group g taking pixels 2t, 2t + 1.
     // x: 3 rows x 32 pixel pairs x 4 groups of 8 input channels, items
     // i = (r, g, t) = thread and thread + 256 (threads 0..127); item i loads
     // columns w0 + 2t - 1 .. w0 + 2t + 2 and writes pixels 2t, 2t + 1 of the
     // three copies (copy dw holds column pp + dw at pixel pp), each
     // channel's pair as one 32-bit write.
-    const bool stage_dy = threadIdx.x < 128, two_items = threadIdx.x < 128;
+    const bool low = threadIdx.x < 128;  // stages dy and takes a second x item
     u4v ldy[2], lm[2], lx[2][4];
     auto load = [&](int64_t sg) {
         const int w0 = (int)(sg % wsegs) * 64;
         const int64_t nh = sg / wsegs;
         const int h = (int)(nh % H);
         const int64_t n = nh / H;
-        if (stage_dy) {
+        if (low) {
             const int t = threadIdx.x & 31, g = threadIdx.x >> 5;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -598,7 +599,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
         }
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
-            if (it == 1 && !two_items) break;
+            if (it == 1 && !low) break;
             const int i = threadIdx.x + 256 * it;
             const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
             const int hh = h + r - 1;
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
         u4v vdy[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) vdy[k] = dym ? relu_keep(ldy[k], lm[k]) : ldy[k];
-        if (stage_dy) {
+        if (low) {
             const int t = threadIdx.x & 31, g = threadIdx.x >> 5;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(256) void conv_bwd_w_small(const uint16_t* __restri
         }
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
-            if (it == 1 && !two_items) break;
+            if (it == 1 && !low) break;
             const int i = threadIdx.x + 256 * it;
             const int t = i & 31, g = (i >> 5) & 3, r = i >> 7;
 #pragma unroll
