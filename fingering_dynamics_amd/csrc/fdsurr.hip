@@ -751,8 +751,11 @@ __device__ __forceinline__ float bf_hi(unsigned v) { return __builtin_bit_cast(f
 // over the 2 x 2 block, taken in the order of the gradients of the two
 // repeats (along w, then along h): the two columns of each row summed and
 // rounded to bf16, then the two rows summed and rounded; dskip =
-// dout[..., C1:].  Pieces [0, n_lo) are dlo's, [n_lo, n_lo + n_skip) dskip's.
-__global__ __launch_bounds__(256) void up_cat_bwd(const uint16_t* __restrict__ dout, uint16_t* __restrict__ dlo,
+// dout[..., C1:].  With lo_y (lo, the output of a ReLU), dlo is zeroed where
+// lo_y is not > 0: that ReLU's backward.  Pieces [0, n_lo) are dlo's,
+// [n_lo, n_lo + n_skip) dskip's.
+__global__ __launch_bounds__(256) void up_cat_bwd(const uint16_t* __restrict__ dout,
+                                                  const uint16_t* __restrict__ lo_y, uint16_t* __restrict__ dlo,
                                                   uint16_t* __restrict__ dskip, int H, int W, int G1, int G2,
                                                   int64_t n_lo, int64_t n_skip) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -776,6 +779,12 @@ __global__ __launch_bounds__(256) void up_cat_bwd(const uint16_t* __restrict__ d
             const float u0 = bf_lo(to_bf16(bf_hi(a[k]) + bf_hi(b[k]))), u1 = bf_lo(to_bf16(bf_hi(c[k]) + bf_hi(e[k])));
             r[k] = (unsigned)to_bf16(t0 + t1) | ((unsigned)to_bf16(u0 + u1) << 16);
         }
+        if (lo_y) {
+            const u4v m = reinterpret_cast<const u4v*>(lo_y)[i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                r[k] &= (bf_lo(m[k]) > 0.f ? 0x0000FFFFu : 0u) | (bf_hi(m[k]) > 0.f ? 0xFFFF0000u : 0u);
+        }
         reinterpret_cast<u4v*>(dlo)[i] = r;
     } else if (i < n_lo + n_skip) {
         const int64_t j = i - n_lo;
@@ -798,6 +807,37 @@ __global__ __launch_bounds__(256) void relu_mask(const uint16_t* __restrict__ dy
         const unsigned lo = bf_lo(m[k]) > 0.f ? 0x0000FFFFu : 0u;
         const unsigned hi = bf_hi(m[k]) > 0.f ? 0xFFFF0000u : 0u;
         r[k] = v[k] & (lo | hi);
+    }
+    reinterpret_cast<u4v*>(out)[i] = r;
+}
+
+// Gradient of a ReLU output s [N][H][W][C] that feeds both a 2 x 2 average
+// pool and a skip connection: out = where(s > 0, g, 0) with g = bf16(pool
+// gradient + dskip), the pool gradient dp[n][h/2][w/2][c] / 4 rounded to
+// bf16 first (the mean's backward), then added to dskip and rounded (the
+// sum of a tensor's two gradients), then masked (the ReLU's backward).
+// One 8-channel piece per thread.
+__global__ __launch_bounds__(256) void pool_skip_bwd(const uint16_t* __restrict__ dp,
+                                                     const uint16_t* __restrict__ dskip,
+                                                     const uint16_t* __restrict__ s, uint16_t* __restrict__ out,
+                                                     int H, int W, int G, int64_t n8) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const int64_t p = i / G;
+    const int g = (int)(i - p * G);
+    const int w = (int)(p % W);
+    const int64_t t = p / W;
+    const int h = (int)(t % H);
+    const int64_t nn = t / H;
+    const int64_t q = (nn * (H / 2) + h / 2) * (W / 2) + w / 2;
+    const u4v a = reinterpret_cast<const u4v*>(dp)[q * G + g];
+    const u4v b = reinterpret_cast<const u4v*>(dskip)[i], m = reinterpret_cast<const u4v*>(s)[i];
+    u4v r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float plo = bf_lo(to_bf16(bf_lo(a[k]) * 0.25f)), phi = bf_lo(to_bf16(bf_hi(a[k]) * 0.25f));
+        const unsigned v = (unsigned)to_bf16(plo + bf_lo(b[k])) | ((unsigned)to_bf16(phi + bf_hi(b[k])) << 16);
+        r[k] = v & ((bf_lo(m[k]) > 0.f ? 0x0000FFFFu : 0u) | (bf_hi(m[k]) > 0.f ? 0xFFFF0000u : 0u));
     }
     reinterpret_cast<u4v*>(out)[i] = r;
 }
@@ -978,12 +1018,17 @@ int fds_up_cat(const uint16_t* lo, const uint16_t* skip, uint16_t* out, int32_t 
 
 int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t N, int32_t H, int32_t W, int32_t C1,
                    int32_t C2, void* stream) {
+    return fds_up_cat_bwd_masked(dout, nullptr, dlo, dskip, N, H, W, C1, C2, stream);
+}
+
+int fds_up_cat_bwd_masked(const uint16_t* dout, const uint16_t* lo, uint16_t* dlo, uint16_t* dskip, int32_t N,
+                          int32_t H, int32_t W, int32_t C1, int32_t C2, void* stream) {
     if (!dout || !dlo || !dskip) return fail(FDS_E_ARG, "NULL pointer");
     if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
     if (C1 < 8 || C2 < 8 || (C1 % 8) || (C2 % 8)) return fail(FDS_E_ARG, "C1, C2 must be positive multiples of 8");
     const int64_t n_lo = (int64_t)N * (H / 2) * (W / 2) * (C1 / 8), n_skip = (int64_t)N * H * W * (C2 / 8);
     hipLaunchKernelGGL(up_cat_bwd, dim3((unsigned)((n_lo + n_skip + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       dout, dlo, dskip, H, W, C1 / 8, C2 / 8, n_lo, n_skip);
+                       dout, lo, dlo, dskip, H, W, C1 / 8, C2 / 8, n_lo, n_skip);
     FDS_HIP(hipGetLastError());
     return FDS_OK;
 }
@@ -993,6 +1038,18 @@ int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t 
     if (n < 8 || (n % 8)) return fail(FDS_E_ARG, "n must be a positive multiple of 8");
     hipLaunchKernelGGL(relu_mask, dim3((unsigned)((n / 8 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dy, y,
                        out, n / 8);
+    FDS_HIP(hipGetLastError());
+    return FDS_OK;
+}
+
+int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, const uint16_t* s, uint16_t* out, int32_t N,
+                      int32_t H, int32_t W, int32_t C, void* stream) {
+    if (!dp || !dskip || !s || !out) return fail(FDS_E_ARG, "NULL pointer");
+    if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
+    if (C < 8 || (C % 8)) return fail(FDS_E_ARG, "C must be a positive multiple of 8");
+    const int64_t n8 = (int64_t)N * H * W * (C / 8);
+    hipLaunchKernelGGL(pool_skip_bwd, dim3((unsigned)((n8 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dp,
+                       dskip, s, out, H, W, C / 8, n8);
     FDS_HIP(hipGetLastError());
     return FDS_OK;
 }

@@ -14,8 +14,12 @@ every convolution but the head: about 1.2M parameters.  Every convolution
 kernels of libfdsurr.so with FP32 accumulation; activations are bf16 NHWC,
 master weights FP32 with AdamW.  The decoder's upsampling + skip
 concatenation and the ReLU backward mask run on small kernels of the same
-library (bit for bit the PyTorch composition they replace); pooling and the
-optimiser are PyTorch tensor operations (plumbing around the convolutions).
+library; the mask of a ReLU output that feeds the upsampling is applied in
+the upsampling's backward, and the gradient of a pooled skip output (pool
+backward, the sum of its two gradients and the mask) is one kernel; all
+are bit for bit the PyTorch composition they replace (FUSE_MASKS = False
+masks every ReLU in its own pass).  Pooling's forward and the optimiser are
+PyTorch tensor operations (plumbing around the convolutions).
 
 Data.  Seeded beds of circles (flow_spec, build_flow) at a fixed size,
 advanced together by one batched launch per iteration; every `snap`
@@ -52,18 +56,21 @@ LOSS_BAND = 0         # > 0: the psi and velocity MSE count only cells within th
                       # the interface at t, the cells a jump may change (0: every fluid cell)
 CHANNELS = (32, 64, 128, 128)
 IN_CH = 8
+FUSE_MASKS = True     # ReLU masks of the skip outputs inside their gradient kernels (False: a separate pass)
 
 
 # --------------------------------------------------------------- network
 class _Conv(torch.autograd.Function):
-    """3x3 convolution on the MFMA kernels; W is the FP32 master weight."""
+    """3x3 convolution on the MFMA kernels; W is the FP32 master weight.
+    dy_masked: the gradient arriving for this convolution's ReLU output is
+    already masked by the kernel that produced it."""
 
     @staticmethod
-    def forward(ctx, x, W, b, relu, out_f32):
+    def forward(ctx, x, W, b, relu, out_f32, dy_masked):
         w16 = W.to(torch.bfloat16).contiguous()
         y = so.conv3x3(x, w16, b, relu=relu, out_dtype=torch.float32 if out_f32 else torch.bfloat16)
-        ctx.save_for_backward(x, w16, y if relu else None)
-        ctx.relu = relu
+        ctx.save_for_backward(x, w16, y if relu and not dy_masked else None)
+        ctx.relu, ctx.dy_masked = relu, dy_masked
         return y
 
     @staticmethod
@@ -71,19 +78,22 @@ class _Conv(torch.autograd.Function):
         x, w16, y = ctx.saved_tensors
         dy = dy.to(torch.bfloat16)
         dy = dy.contiguous()
-        if ctx.relu:
+        if ctx.relu and not ctx.dy_masked:
             # one masking pass: measured faster here than the kernels'
-            # mask-on-load (relu_y), which adds a load per operand fragment
+            # mask-on-load (relu_y), which adds a load per operand fragment,
+            # and than a mask in the input-gradient epilogue of the
+            # convolution consuming y, whose scattered 2-byte mask loads
+            # slowed those kernels by more than the pass costs
             dy = so.relu_mask(dy, y)
         dx = None
         if ctx.needs_input_grad[0]:
             dx = so.conv3x3_bwd_data(dy, so.flip_weights(w16))
         dw, db = so.conv3x3_bwd_weight(x, dy)
-        return dx, dw, db, None, None
+        return dx, dw, db, None, None, None
 
 
-def conv(x, W, b, relu=True, out_f32=False):
-    return _Conv.apply(x, W, b, relu, out_f32)
+def conv(x, W, b, relu=True, out_f32=False, dy_masked=False):
+    return _Conv.apply(x, W, b, relu, out_f32, dy_masked)
 
 
 def _pool(x):
@@ -97,20 +107,47 @@ def _up(x):
 
 class _UpCat(torch.autograd.Function):
     """torch.cat([_up(lo), skip], dim=3) on one kernel each way, bit for
-    bit the same as that composition and its autograd."""
+    bit the same as that composition and its autograd.  mask_lo: lo is a
+    ReLU's bf16 output that feeds only this upsampling, and its gradient
+    takes that ReLU's mask in the same kernel."""
 
     @staticmethod
-    def forward(ctx, lo, skip):
+    def forward(ctx, lo, skip, mask_lo):
         ctx.c1 = lo.shape[3]
+        ctx.save_for_backward(lo if mask_lo else None)
         return so.up_cat(lo.contiguous(), skip.contiguous())
 
     @staticmethod
     def backward(ctx, dout):
-        return so.up_cat_bwd(dout.to(torch.bfloat16).contiguous(), ctx.c1)
+        (lo,) = ctx.saved_tensors
+        dlo, dskip = so.up_cat_bwd(dout.to(torch.bfloat16).contiguous(), ctx.c1,
+                                   lo_relu=None if lo is None else lo.contiguous())
+        return dlo, dskip, None
 
 
-def up_cat(lo, skip):
-    return _UpCat.apply(lo, skip)
+def up_cat(lo, skip, mask_lo=False):
+    return _UpCat.apply(lo, skip, mask_lo)
+
+
+class _PoolSkip(torch.autograd.Function):
+    """(_pool(s), s) for a ReLU's bf16 output s that feeds a pooling and a
+    skip connection: the backward forms s's gradient - the pool's backward,
+    the sum of the two gradients and the ReLU's mask - in one kernel, bit
+    for bit that composition."""
+
+    @staticmethod
+    def forward(ctx, s):
+        ctx.save_for_backward(s)
+        return _pool(s), s.view_as(s)
+
+    @staticmethod
+    def backward(ctx, dp, dskip):
+        (s,) = ctx.saved_tensors
+        return so.pool_skip_bwd(dp.to(torch.bfloat16).contiguous(), dskip.to(torch.bfloat16).contiguous(), s)
+
+
+def pool_skip(s):
+    return _PoolSkip.apply(s)
 
 
 class UNet(torch.nn.Module):
@@ -127,19 +164,32 @@ class UNet(torch.nn.Module):
             self.register_parameter("w_" + name, torch.nn.Parameter(torch.randn(cout, 3, 3, cin, generator=g) * std))
             self.register_parameter("b_" + name, torch.nn.Parameter(torch.zeros(cout)))
 
-    def _c(self, name, x, relu=True, out_f32=False):
-        return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32)
+    def _c(self, name, x, relu=True, out_f32=False, dy_masked=False):
+        return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32, dy_masked)
 
     def forward(self, x):
         """x [N, H, W, 8] bf16 (H, W multiples of 8) -> [N, H, W, 3] fp32:
         the change over K iterations of psi, ux / U_REF and uy / U_REF."""
-        s0 = self._c("e0b", self._c("e0a", x))
-        s1 = self._c("e1b", self._c("e1a", _pool(s0)))
-        s2 = self._c("e2b", self._c("e2a", _pool(s1)))
-        b = self._c("e3b", self._c("e3a", _pool(s2)))
-        d = self._c("d2b", self._c("d2a", up_cat(b, s2)))
-        d = self._c("d1b", self._c("d1a", up_cat(d, s1)))
-        d = self._c("d0b", self._c("d0a", up_cat(d, s0)))
+        if not FUSE_MASKS:
+            s0 = self._c("e0b", self._c("e0a", x))
+            s1 = self._c("e1b", self._c("e1a", _pool(s0)))
+            s2 = self._c("e2b", self._c("e2a", _pool(s1)))
+            b = self._c("e3b", self._c("e3a", _pool(s2)))
+            d = self._c("d2b", self._c("d2a", up_cat(b, s2)))
+            d = self._c("d1b", self._c("d1a", up_cat(d, s1)))
+            d = self._c("d0b", self._c("d0a", up_cat(d, s0)))
+            return self._c("head", d, relu=False, out_f32=True)[..., :3]
+        # the gradients of the level outputs are masked by the kernels that
+        # produce them: the pooled skip's backward (pool_skip) or the
+        # upsampling's (mask_lo); the other ReLUs mask in their own pass
+        m = dict(dy_masked=True)
+        p0, s0 = pool_skip(self._c("e0b", self._c("e0a", x), **m))
+        p1, s1 = pool_skip(self._c("e1b", self._c("e1a", p0), **m))
+        p2, s2 = pool_skip(self._c("e2b", self._c("e2a", p1), **m))
+        b = self._c("e3b", self._c("e3a", p2), **m)
+        d = self._c("d2b", self._c("d2a", up_cat(b, s2, mask_lo=True)), **m)
+        d = self._c("d1b", self._c("d1a", up_cat(d, s1, mask_lo=True)), **m)
+        d = self._c("d0b", self._c("d0a", up_cat(d, s0, mask_lo=True)))
         return self._c("head", d, relu=False, out_f32=True)[..., :3]
 
     def n_params(self):

@@ -70,10 +70,25 @@ int fds_up_cat(const uint16_t* lo, const uint16_t* skip, uint16_t* out, int32_t 
 int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t N, int32_t H, int32_t W,
                    int32_t C1, int32_t C2, void* stream);
 
+/* fds_up_cat_bwd with dlo zeroed where lo (the coarse input, the bf16 output
+ * of a ReLU, N x H/2 x W/2 x C1) is not > 0: that ReLU's backward.  lo may
+ * be NULL (then fds_up_cat_bwd). */
+int fds_up_cat_bwd_masked(const uint16_t* dout, const uint16_t* lo, uint16_t* dlo, uint16_t* dskip, int32_t N,
+                          int32_t H, int32_t W, int32_t C1, int32_t C2, void* stream);
+
 /* out = dy where y > 0 (bf16 values as floats: -0 and NaN are not), else 0:
  * the backward mask of a ReLU whose bf16 output is y; n values, n % 8 == 0;
  * out may be dy. */
 int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t n, void* stream);
+
+/* Gradient of s (N x H x W x C bf16, H and W even, C a multiple of 8), the
+ * output of a ReLU that feeds a 2 x 2 average pool (gradient dp, N x H/2 x
+ * W/2 x C) and a skip connection (gradient dskip, N x H x W x C):
+ * out = 0 where s is not > 0, else bf16(bf16(dp[n][h/2][w/2][c] / 4) +
+ * dskip[n][h][w][c]) -- the mean's backward, the sum of the two gradients
+ * and the ReLU's backward, each rounded as torch rounds them. */
+int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, const uint16_t* s, uint16_t* out, int32_t N,
+                      int32_t H, int32_t W, int32_t C, void* stream);
 
 /* Message of the last failed call on this thread. */
 const char* fds_last_error(void);

@@ -135,9 +135,11 @@ def test_relu_mask_is_torch_where():
     assert torch.equal(so.relu_mask(dy, y), ref)
 
 
-def test_unet_gradients_unchanged_by_the_plumbing_kernels(monkeypatch):
+@pytest.mark.parametrize("fuse", [True, False])
+def test_unet_gradients_unchanged_by_the_plumbing_kernels(monkeypatch, fuse):
     """A training step of the U-Net gives the same loss and gradients, bit for
-    bit, with the up-cat and ReLU-mask kernels as with the PyTorch
+    bit, with the up-cat kernels and the skip outputs' ReLU masks inside
+    their gradient kernels (fuse) or in their own pass, as with the PyTorch
     operations they replace."""
     from fingering_dynamics_amd import surrogate as sg
     from fingering_dynamics_amd import surrogate_ops as so
@@ -151,11 +153,59 @@ def test_unet_gradients_unchanged_by_the_plumbing_kernels(monkeypatch):
         loss.backward()
         return float(loss), [p.grad.clone() for p in model.parameters()]
 
+    monkeypatch.setattr(sg, "FUSE_MASKS", fuse)
     loss_k, g_k = grads()
-    monkeypatch.setattr(sg, "up_cat", lambda lo, skip: torch.cat([sg._up(lo), skip], dim=3).contiguous())
+    monkeypatch.setattr(sg, "FUSE_MASKS", False)
+    monkeypatch.setattr(sg, "up_cat", lambda lo, skip, **kw: torch.cat([sg._up(lo), skip], dim=3).contiguous())
     monkeypatch.setattr(so, "relu_mask",
                         lambda dy, y: torch.where(y > 0, dy, torch.zeros((), dtype=dy.dtype, device=dy.device)))
     loss_t, g_t = grads()
     assert loss_k == loss_t
     for a, b in zip(g_k, g_t):
         assert torch.equal(a, b)
+
+
+def _torch_mask(dy, y):
+    return torch.where(y > 0, dy, torch.zeros((), dtype=dy.dtype, device=dy.device))
+
+
+def _spread(shape, gen):
+    """bf16 values over about 60 binades with both signs and exact zeros, so
+    that rounding and summation order show."""
+    m = torch.randn(shape, generator=gen)
+    e = torch.randint(-30, 30, shape, generator=gen).float()
+    v = m * torch.pow(2.0, e)
+    v[torch.rand(shape, generator=gen) < 0.05] = 0.0
+    return v.to(torch.bfloat16).cuda()
+
+
+def test_up_cat_backward_with_the_coarse_mask():
+    """up_cat_bwd(dout, c1, lo_relu=lo): dlo is relu_mask(dlo, lo) bit for
+    bit and dskip unchanged."""
+    from fingering_dynamics_amd import surrogate_ops as so
+    gen = torch.Generator().manual_seed(5)
+    lo = torch.relu(torch.randn(2, 8, 16, 64, generator=gen)).to(torch.bfloat16).cuda()
+    dout = _spread((2, 16, 32, 96), gen)
+    dlo_m, dskip_m = so.up_cat_bwd(dout, 64, lo_relu=lo)
+    dlo, dskip = so.up_cat_bwd(dout, 64)
+    assert torch.equal(dlo_m, _torch_mask(dlo, lo))
+    assert torch.equal(dskip_m, dskip)
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 32, 32), (3, 8, 8, 128), (1, 2, 2, 8)])
+def test_pool_skip_backward_is_the_torch_composition(shape):
+    """pool_skip_bwd(dp, dskip, s) is, bit for bit, torch's gradient of s
+    through its 2 x 2 mean pool and its identity (autograd's sum of the two
+    bf16 gradients), masked by s > 0, on values over about 60 binades."""
+    from fingering_dynamics_amd import surrogate as sg
+    from fingering_dynamics_amd import surrogate_ops as so
+    N, H, W, C = shape
+    gen = torch.Generator().manual_seed(H * W + C)
+    s = _spread(shape, gen)
+    dp = _spread((N, H // 2, W // 2, C), gen)
+    dskip = _spread(shape, gen)
+    leaf = s.clone().requires_grad_(True)
+    (g,) = torch.autograd.grad([sg._pool(leaf), leaf.view_as(leaf)], [leaf], [dp, dskip])
+    want = _torch_mask(g, s)
+    got = so.pool_skip_bwd(dp, dskip, s)
+    assert torch.equal(got, want)
