@@ -61,21 +61,22 @@ FUSE_MASKS = True     # ReLU masks of the skip outputs inside their gradient ker
 
 # --------------------------------------------------------------- network
 class _Conv(torch.autograd.Function):
-    """3x3 convolution on the MFMA kernels; W is the FP32 master weight.
-    dy_masked: the gradient arriving for this convolution's ReLU output is
-    already masked by the kernel that produced it."""
+    """3x3 convolution on the MFMA kernels; W is the FP32 master weight,
+    w16 and wt its bf16 rounding and that flipped for the input gradient
+    (cast_flip of the network's weights).  dy_masked: the gradient arriving
+    for this convolution's ReLU output is already masked by the kernel that
+    produced it."""
 
     @staticmethod
-    def forward(ctx, x, W, b, relu, out_f32, dy_masked):
-        w16 = W.to(torch.bfloat16).contiguous()
+    def forward(ctx, x, W, b, relu, out_f32, dy_masked, w16, wt):
         y = so.conv3x3(x, w16, b, relu=relu, out_dtype=torch.float32 if out_f32 else torch.bfloat16)
-        ctx.save_for_backward(x, w16, y if relu and not dy_masked else None)
+        ctx.save_for_backward(x, wt, y if relu and not dy_masked else None)
         ctx.relu, ctx.dy_masked = relu, dy_masked
         return y
 
     @staticmethod
     def backward(ctx, dy):
-        x, w16, y = ctx.saved_tensors
+        x, wt, y = ctx.saved_tensors
         dy = dy.to(torch.bfloat16)
         dy = dy.contiguous()
         if ctx.relu and not ctx.dy_masked:
@@ -87,13 +88,15 @@ class _Conv(torch.autograd.Function):
             dy = so.relu_mask(dy, y)
         dx = None
         if ctx.needs_input_grad[0]:
-            dx = so.conv3x3_bwd_data(dy, so.flip_weights(w16))
+            dx = so.conv3x3_bwd_data(dy, wt)
         dw, db = so.conv3x3_bwd_weight(x, dy)
-        return dx, dw, db, None, None, None
+        return dx, dw, db, None, None, None, None, None
 
 
-def conv(x, W, b, relu=True, out_f32=False, dy_masked=False):
-    return _Conv.apply(x, W, b, relu, out_f32, dy_masked)
+def conv(x, W, b, relu=True, out_f32=False, dy_masked=False, w16=None, wt=None):
+    if w16 is None:
+        (w16,), (wt,) = so.cast_flip([W.detach()])
+    return _Conv.apply(x, W, b, relu, out_f32, dy_masked, w16, wt)
 
 
 def _pool(x):
@@ -165,11 +168,15 @@ class UNet(torch.nn.Module):
             self.register_parameter("b_" + name, torch.nn.Parameter(torch.zeros(cout)))
 
     def _c(self, name, x, relu=True, out_f32=False, dy_masked=False):
-        return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32, dy_masked)
+        w16, wt = self._w[name]
+        return conv(x, getattr(self, "w_" + name), getattr(self, "b_" + name), relu, out_f32, dy_masked, w16, wt)
 
     def forward(self, x):
         """x [N, H, W, 8] bf16 (H, W multiples of 8) -> [N, H, W, 3] fp32:
         the change over K iterations of psi, ux / U_REF and uy / U_REF."""
+        # every layer's bf16 weights and flipped weights in one launch
+        w16, wt = so.cast_flip([getattr(self, "w_" + n).detach() for n in self.names])
+        self._w = dict(zip(self.names, zip(w16, wt)))
         if not FUSE_MASKS:
             s0 = self._c("e0b", self._c("e0a", x))
             s1 = self._c("e1b", self._c("e1a", _pool(s0)))

@@ -18,6 +18,7 @@ _VP, _I32 = ctypes.c_void_p, ctypes.c_int32
 EXPORTS = {
     "fds_conv3x3_fwd": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 6 + [_VP]),
     "fds_flip_weights": (ctypes.c_int, [_VP, _VP, _I32, _I32, _VP]),
+    "fds_cast_flip": (ctypes.c_int, [_I32, _VP, _VP, _VP, _VP, _VP, _VP]),
     "fds_conv3x3_bwd_data": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
     "fds_bwd_weight_scratch": (ctypes.c_int64, [_I32] * 5),
     "fds_conv3x3_bwd_weight": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP] + [_I32] * 5 + [_VP]),
@@ -112,6 +113,26 @@ def _mask(relu_y, dy):
                                or not relu_y.is_contiguous()):
         raise ValueError("relu_y must be a contiguous bf16 tensor shaped like dy")
     return relu_y
+
+
+def cast_flip(ws):
+    """([w.to(bf16)], [flip_weights(w.to(bf16))]) for a list of at most 16
+    contiguous float32 weights [Cout, 3, 3, Cin] on one GPU, in one kernel
+    launch."""
+    n = len(ws)
+    if not 1 <= n <= 16:
+        raise ValueError("1 to 16 weights expected")
+    for w in ws:
+        if w.dtype != torch.float32 or not w.is_contiguous() or w.dim() != 4 or tuple(w.shape[1:3]) != (3, 3):
+            raise ValueError("weights must be contiguous float32 [Cout, 3, 3, Cin] tensors")
+    _same_device(*ws)
+    w16 = [torch.empty(w.shape, dtype=torch.bfloat16, device=w.device) for w in ws]
+    wt = [torch.empty((w.shape[3], 3, 3, w.shape[0]), dtype=torch.bfloat16, device=w.device) for w in ws]
+    ptrs = lambda ts: (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+    ints = lambda vs: (ctypes.c_int32 * n)(*vs)
+    _check(load().fds_cast_flip(n, ptrs(ws), ptrs(w16), ptrs(wt), ints([w.shape[3] for w in ws]),
+                                ints([w.shape[0] for w in ws]), _stream()))
+    return w16, wt
 
 
 def conv3x3_bwd_data(dy, wt, out_dtype=torch.bfloat16, relu_y=None):

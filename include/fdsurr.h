@@ -37,6 +37,13 @@ int fds_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, flo
  * convolution, so that conv3x3(dy, wt) is the gradient of the input. */
 int fds_flip_weights(const uint16_t* w, uint16_t* wt, int32_t Cin, int32_t Cout, void* stream);
 
+/* For n <= 16 layers at once, in one launch: w16[l] = w[l] rounded to bf16
+ * (nearest even) and wt[l] = fds_flip_weights of it, from the FP32 weights
+ * w[l] [cout[l]][3][3][cin[l]].  w, w16, wt, cin and cout are host arrays of
+ * n entries (device pointers in the first three). */
+int fds_cast_flip(int32_t n, const float* const* w, uint16_t* const* w16, uint16_t* const* wt, const int32_t* cin,
+                  const int32_t* cout, void* stream);
+
 /* dx = conv3x3(dy', wt) with wt from fds_flip_weights (dy has Cout
  * channels, dx has Cin).  dy' = dy where relu_y > 0 and 0 elsewhere when
  * relu_y (the bf16 output of a ReLU forward, same shape as dy) is given,

@@ -134,6 +134,9 @@ def test_surrogate_library_rejects_bad_arguments_before_any_launch():
         (lambda: lib.fds_conv3x3_bwd_weight(None, None, None, None, None, None, 1, 8, 8, 8, 8, None),
          "NULL pointer"),
         (lambda: lib.fds_flip_weights(None, None, 8, 8, None), "bad arguments"),
+        (lambda: lib.fds_cast_flip(0, None, None, None, None, None, None), "1 to 16 layers"),
+        (lambda: lib.fds_cast_flip(17, None, None, None, None, None, None), "1 to 16 layers"),
+        (lambda: lib.fds_cast_flip(1, None, None, None, None, None, None), "NULL pointer"),
         # a non-NULL address that no call below dereferences: the shape
         # checks reject it first
         (lambda: lib.fds_up_cat(None, bogus, bogus, 1, 8, 8, 8, 8, None), "NULL pointer"),

@@ -75,6 +75,22 @@ def test_conv_forward_backward(shape):
         float((dw.double().cpu() - dw_ref).abs().max()), float((db.double().cpu() - db_ref).abs().max())))
 
 
+def test_cast_flip_is_the_bf16_cast_and_flip_weights():
+    """cast_flip of several layers in one launch: each w.to(bf16) and
+    flip_weights of it, bit for bit (round to nearest even, ties included)."""
+    from fingering_dynamics_amd import surrogate_ops as so
+    g = torch.Generator().manual_seed(9)
+    shapes = [(32, 8), (32, 32), (64, 96), (8, 32), (128, 256)]
+    ws = [(torch.randn(co, 3, 3, ci, generator=g) * 0.3).cuda() for co, ci in shapes]
+    # exact ties between two bf16 values, which round to the even one
+    ws[0].view(-1)[:4] = torch.tensor([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 2.0 ** -100 * 1.00390625])
+    w16, wt = so.cast_flip(ws)
+    for w, a, b in zip(ws, w16, wt):
+        ref = w.to(torch.bfloat16)
+        assert torch.equal(a.view(torch.int16), ref.view(torch.int16))
+        assert torch.equal(b, so.flip_weights(ref))
+
+
 def test_flip_weights_is_an_involution_with_transpose():
     from fingering_dynamics_amd import surrogate_ops as so
     g = torch.Generator().manual_seed(5)
