@@ -845,9 +845,10 @@ __global__ __launch_bounds__(256) void relu_mask(const uint16_t* __restrict__ dy
 // gradient + dskip), the pool gradient dp[n][h/2][w/2][c] / 4 rounded to
 // bf16 first (the mean's backward), then added to dskip and rounded (the
 // sum of a tensor's two gradients), then masked (the ReLU's backward).
-// One 8-channel piece per thread.
+// dskip's pixels are Gs pieces of 8 channels apart (a channel slice of a
+// wider gradient may be read in place).  One 8-channel piece per thread.
 __global__ __launch_bounds__(256) void pool_skip_bwd(const uint16_t* __restrict__ dp,
-                                                     const uint16_t* __restrict__ dskip,
+                                                     const uint16_t* __restrict__ dskip, int Gs,
                                                      const uint16_t* __restrict__ s, uint16_t* __restrict__ out,
                                                      int H, int W, int G, int64_t n8) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -860,7 +861,7 @@ __global__ __launch_bounds__(256) void pool_skip_bwd(const uint16_t* __restrict_
     const int64_t nn = t / H;
     const int64_t q = (nn * (H / 2) + h / 2) * (W / 2) + w / 2;
     const u4v a = reinterpret_cast<const u4v*>(dp)[q * G + g];
-    const u4v b = reinterpret_cast<const u4v*>(dskip)[i], m = reinterpret_cast<const u4v*>(s)[i];
+    const u4v b = reinterpret_cast<const u4v*>(dskip)[p * Gs + g], m = reinterpret_cast<const u4v*>(s)[i];
     u4v r;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1069,15 +1070,16 @@ int fds_up_cat(const uint16_t* lo, const uint16_t* skip, uint16_t* out, int32_t 
 
 int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t N, int32_t H, int32_t W, int32_t C1,
                    int32_t C2, void* stream) {
+    if (!dskip) return fail(FDS_E_ARG, "NULL pointer");
     return fds_up_cat_bwd_masked(dout, nullptr, dlo, dskip, N, H, W, C1, C2, stream);
 }
 
 int fds_up_cat_bwd_masked(const uint16_t* dout, const uint16_t* lo, uint16_t* dlo, uint16_t* dskip, int32_t N,
                           int32_t H, int32_t W, int32_t C1, int32_t C2, void* stream) {
-    if (!dout || !dlo || !dskip) return fail(FDS_E_ARG, "NULL pointer");
+    if (!dout || !dlo) return fail(FDS_E_ARG, "NULL pointer");
     if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
     if (C1 < 8 || C2 < 8 || (C1 % 8) || (C2 % 8)) return fail(FDS_E_ARG, "C1, C2 must be positive multiples of 8");
-    const int64_t n_lo = (int64_t)N * (H / 2) * (W / 2) * (C1 / 8), n_skip = (int64_t)N * H * W * (C2 / 8);
+    const int64_t n_lo = (int64_t)N * (H / 2) * (W / 2) * (C1 / 8), n_skip = dskip ? (int64_t)N * H * W * (C2 / 8) : 0;
     hipLaunchKernelGGL(up_cat_bwd, dim3((unsigned)((n_lo + n_skip + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        dout, lo, dlo, dskip, H, W, C1 / 8, C2 / 8, n_lo, n_skip);
     FDS_HIP(hipGetLastError());
@@ -1093,14 +1095,15 @@ int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t 
     return FDS_OK;
 }
 
-int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, const uint16_t* s, uint16_t* out, int32_t N,
-                      int32_t H, int32_t W, int32_t C, void* stream) {
+int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, int32_t dskip_ld, const uint16_t* s, uint16_t* out,
+                      int32_t N, int32_t H, int32_t W, int32_t C, void* stream) {
     if (!dp || !dskip || !s || !out) return fail(FDS_E_ARG, "NULL pointer");
     if (N < 1 || H < 2 || W < 2 || (H % 2) || (W % 2)) return fail(FDS_E_ARG, "N >= 1 and even H, W >= 2 expected");
     if (C < 8 || (C % 8)) return fail(FDS_E_ARG, "C must be a positive multiple of 8");
+    if (dskip_ld < C || (dskip_ld % 8)) return fail(FDS_E_ARG, "dskip_ld must be a multiple of 8, at least C");
     const int64_t n8 = (int64_t)N * H * W * (C / 8);
     hipLaunchKernelGGL(pool_skip_bwd, dim3((unsigned)((n8 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dp,
-                       dskip, s, out, H, W, C / 8, n8);
+                       dskip, dskip_ld / 8, s, out, H, W, C / 8, n8);
     FDS_HIP(hipGetLastError());
     return FDS_OK;
 }

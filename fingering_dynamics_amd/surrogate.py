@@ -123,8 +123,10 @@ class _UpCat(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dout):
         (lo,) = ctx.saved_tensors
+        # with mask_lo the skip is a pooled skip output, whose backward reads
+        # its gradient in place from dout's channels
         dlo, dskip = so.up_cat_bwd(dout.to(torch.bfloat16).contiguous(), ctx.c1,
-                                   lo_relu=None if lo is None else lo.contiguous())
+                                   lo_relu=None if lo is None else lo.contiguous(), skip_copy=lo is None)
         return dlo, dskip, None
 
 
@@ -146,7 +148,9 @@ class _PoolSkip(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dp, dskip):
         (s,) = ctx.saved_tensors
-        return so.pool_skip_bwd(dp.to(torch.bfloat16).contiguous(), dskip.to(torch.bfloat16).contiguous(), s)
+        if dskip.dtype != torch.bfloat16 or dskip.stride(3) != 1:
+            dskip = dskip.to(torch.bfloat16).contiguous()
+        return so.pool_skip_bwd(dp.to(torch.bfloat16).contiguous(), dskip, s)
 
 
 def pool_skip(s):

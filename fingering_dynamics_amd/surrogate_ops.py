@@ -26,7 +26,7 @@ EXPORTS = {
     "fds_up_cat_bwd": (ctypes.c_int, [_VP, _VP, _VP] + [_I32] * 5 + [_VP]),
     "fds_up_cat_bwd_masked": (ctypes.c_int, [_VP] * 4 + [_I32] * 5 + [_VP]),
     "fds_relu_mask": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _VP]),
-    "fds_pool_skip_bwd": (ctypes.c_int, [_VP] * 4 + [_I32] * 4 + [_VP]),
+    "fds_pool_skip_bwd": (ctypes.c_int, [_VP, _VP, _I32, _VP, _VP] + [_I32] * 4 + [_VP]),
     "fds_last_error": (ctypes.c_char_p, []),
 }
 
@@ -188,12 +188,13 @@ def up_cat(lo, skip):
     return out
 
 
-def up_cat_bwd(dout, c1, lo_relu=None):
+def up_cat_bwd(dout, c1, lo_relu=None, skip_copy=True):
     """(dlo, dskip): the gradients of up_cat's inputs from that of its
     output, dout [N, H, W, C1 + C2] bf16 with C1 = c1; dlo sums each 2 x 2
     block as the gradients of two nearest-neighbour repeats do (columns,
     then rows, each rounded to bf16).  lo_relu (lo, when it is a ReLU's bf16
-    output) zeroes dlo where lo is not > 0: that ReLU's backward."""
+    output) zeroes dlo where lo is not > 0: that ReLU's backward.  Without
+    skip_copy, dskip is the view dout[..., C1:] (no copy)."""
     N, H, W, C = _nhwc(dout, torch.bfloat16)
     c1 = int(c1)
     if H % 2 or W % 2 or c1 % 8 or c1 < 8 or C - c1 < 8 or (C - c1) % 8:
@@ -203,10 +204,10 @@ def up_cat_bwd(dout, c1, lo_relu=None):
         raise ValueError("lo_relu must be a contiguous bf16 [N, H/2, W/2, C1] tensor")
     _same_device(dout, lo_relu)
     dlo = torch.empty((N, H // 2, W // 2, c1), dtype=torch.bfloat16, device=dout.device)
-    dskip = torch.empty((N, H, W, C - c1), dtype=torch.bfloat16, device=dout.device)
+    dskip = torch.empty((N, H, W, C - c1), dtype=torch.bfloat16, device=dout.device) if skip_copy else None
     _check(load().fds_up_cat_bwd_masked(_p(dout), _p(lo_relu), _p(dlo), _p(dskip), N, H, W, c1, C - c1,
                                         _stream()))
-    return dlo, dskip
+    return dlo, (dskip if skip_copy else dout[..., c1:])
 
 
 def relu_mask(dy, y):
@@ -225,17 +226,21 @@ def relu_mask(dy, y):
 def pool_skip_bwd(dp, dskip, s):
     """Gradient of s [N, H, W, C] bf16, a ReLU output feeding both a 2 x 2
     average pool (output gradient dp [N, H/2, W/2, C]) and a skip connection
-    (gradient dskip [N, H, W, C]): relu_mask(bf16(dp / 4) upsampled + dskip,
-    s), rounded as torch's mean backward, gradient sum and mask round it,
-    in one kernel."""
+    (gradient dskip [N, H, W, C], contiguous or a channel slice of a
+    contiguous [N, H, W, C'] tensor): relu_mask(bf16(dp / 4) upsampled +
+    dskip, s), rounded as torch's mean backward, gradient sum and mask round
+    it, in one kernel."""
     N, H, W, C = _nhwc(s, torch.bfloat16)
-    _nhwc(dskip, torch.bfloat16)
     _nhwc(dp, torch.bfloat16)
     if H % 2 or W % 2 or C % 8:
         raise ValueError("s must be [N, H, W, C] with H, W even and C a multiple of 8")
     if tuple(dskip.shape) != (N, H, W, C) or tuple(dp.shape) != (N, H // 2, W // 2, C):
         raise ValueError("dskip must be shaped like s and dp like its 2 x 2 pool")
+    ld = dskip.stride(2)
+    if (dskip.dtype != torch.bfloat16 or dskip.stride(3) != 1 or ld < C or ld % 8
+            or dskip.stride(1) != W * ld or dskip.stride(0) != H * W * ld or dskip.storage_offset() % 8):
+        raise ValueError("dskip must be bf16 with contiguous channels, pixels a multiple of 8 apart")
     _same_device(s, dskip, dp)
     out = torch.empty_like(s)
-    _check(load().fds_pool_skip_bwd(_p(dp), _p(dskip), _p(s), _p(out), N, H, W, C, _stream()))
+    _check(load().fds_pool_skip_bwd(_p(dp), _p(dskip), ld, _p(s), _p(out), N, H, W, C, _stream()))
     return out

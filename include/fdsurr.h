@@ -79,7 +79,9 @@ int fds_up_cat_bwd(const uint16_t* dout, uint16_t* dlo, uint16_t* dskip, int32_t
 
 /* fds_up_cat_bwd with dlo zeroed where lo (the coarse input, the bf16 output
  * of a ReLU, N x H/2 x W/2 x C1) is not > 0: that ReLU's backward.  lo may
- * be NULL (then fds_up_cat_bwd). */
+ * be NULL (then fds_up_cat_bwd).  dskip may be NULL: it is then not formed,
+ * and the caller reads the skip gradient as the channels C1 .. C1 + C2 - 1
+ * of dout in place. */
 int fds_up_cat_bwd_masked(const uint16_t* dout, const uint16_t* lo, uint16_t* dlo, uint16_t* dskip, int32_t N,
                           int32_t H, int32_t W, int32_t C1, int32_t C2, void* stream);
 
@@ -90,12 +92,14 @@ int fds_relu_mask(const uint16_t* dy, const uint16_t* y, uint16_t* out, int64_t 
 
 /* Gradient of s (N x H x W x C bf16, H and W even, C a multiple of 8), the
  * output of a ReLU that feeds a 2 x 2 average pool (gradient dp, N x H/2 x
- * W/2 x C) and a skip connection (gradient dskip, N x H x W x C):
+ * W/2 x C) and a skip connection (gradient dskip, N x H x W x C with pixels
+ * dskip_ld channels apart, dskip_ld >= C a multiple of 8, so that a channel
+ * slice of the decoder input's gradient is read in place):
  * out = 0 where s is not > 0, else bf16(bf16(dp[n][h/2][w/2][c] / 4) +
  * dskip[n][h][w][c]) -- the mean's backward, the sum of the two gradients
  * and the ReLU's backward, each rounded as torch rounds them. */
-int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, const uint16_t* s, uint16_t* out, int32_t N,
-                      int32_t H, int32_t W, int32_t C, void* stream);
+int fds_pool_skip_bwd(const uint16_t* dp, const uint16_t* dskip, int32_t dskip_ld, const uint16_t* s, uint16_t* out,
+                      int32_t N, int32_t H, int32_t W, int32_t C, void* stream);
 
 /* Message of the last failed call on this thread. */
 const char* fds_last_error(void);

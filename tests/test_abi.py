@@ -147,11 +147,13 @@ def test_surrogate_library_rejects_bad_arguments_before_any_launch():
         (lambda: lib.fds_up_cat_bwd(bogus, bogus, bogus, 1, 8, 8, 8, 0, None), "multiples of 8"),
         (lambda: lib.fds_relu_mask(bogus, None, bogus, 8, None), "NULL pointer"),
         (lambda: lib.fds_relu_mask(bogus, bogus, bogus, 12, None), "multiple of 8"),
-        (lambda: lib.fds_up_cat_bwd_masked(bogus, bogus, bogus, None, 1, 8, 8, 8, 8, None), "NULL pointer"),
+        (lambda: lib.fds_up_cat_bwd_masked(bogus, bogus, None, bogus, 1, 8, 8, 8, 8, None), "NULL pointer"),
         (lambda: lib.fds_up_cat_bwd_masked(bogus, bogus, bogus, bogus, 1, 8, 9, 8, 8, None), "even H, W"),
-        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, None, bogus, 1, 8, 8, 8, None), "NULL pointer"),
-        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, bogus, bogus, 1, 8, 7, 8, None), "even H, W"),
-        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, bogus, bogus, 1, 8, 8, 12, None), "multiple of 8"),
+        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, 8, None, bogus, 1, 8, 8, 8, None), "NULL pointer"),
+        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, 8, bogus, bogus, 1, 8, 7, 8, None), "even H, W"),
+        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, 16, bogus, bogus, 1, 8, 8, 12, None), "multiple of 8"),
+        (lambda: lib.fds_pool_skip_bwd(bogus, bogus, 12, bogus, bogus, 1, 8, 8, 8, None), "dskip_ld"),
+        (lambda: lib.fds_up_cat_bwd(bogus, bogus, None, 1, 8, 8, 8, 8, None), "NULL pointer"),
     ]
     for call, msg in cases:
         assert call() == -1

@@ -206,6 +206,9 @@ def test_up_cat_backward_with_the_coarse_mask():
     dlo, dskip = so.up_cat_bwd(dout, 64)
     assert torch.equal(dlo_m, _torch_mask(dlo, lo))
     assert torch.equal(dskip_m, dskip)
+    dlo_v, dskip_v = so.up_cat_bwd(dout, 64, lo_relu=lo, skip_copy=False)
+    assert torch.equal(dlo_v, dlo_m) and torch.equal(dskip_v, dskip)
+    assert dskip_v.data_ptr() == dout.data_ptr() + 64 * dout.element_size()
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 32, 32), (3, 8, 8, 128), (1, 2, 2, 8)])
@@ -225,3 +228,6 @@ def test_pool_skip_backward_is_the_torch_composition(shape):
     want = _torch_mask(g, s)
     got = so.pool_skip_bwd(dp, dskip, s)
     assert torch.equal(got, want)
+    # the skip gradient read in place as a channel slice of a wider tensor
+    wide = torch.cat([_spread((N, H, W, 16), gen), dskip, _spread((N, H, W, 8), gen)], dim=3).contiguous()
+    assert torch.equal(so.pool_skip_bwd(dp, wide[..., 16:16 + C], s), want)
