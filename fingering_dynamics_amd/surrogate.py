@@ -148,7 +148,7 @@ class _PoolSkip(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dp, dskip):
         (s,) = ctx.saved_tensors
-        if dskip.dtype != torch.bfloat16 or dskip.stride(3) != 1:
+        if so.channel_slice_ld(dskip) is None:
             dskip = dskip.to(torch.bfloat16).contiguous()
         return so.pool_skip_bwd(dp.to(torch.bfloat16).contiguous(), dskip, s)
 

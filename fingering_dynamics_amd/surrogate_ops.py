@@ -223,6 +223,21 @@ def relu_mask(dy, y):
     return out
 
 
+def channel_slice_ld(t):
+    """Pixel stride (in channels) of t [N, H, W, C] when t is bf16 with
+    contiguous channels and pixels a multiple of 8 channels apart (a
+    contiguous tensor, or a channel slice of one at an offset that is a
+    multiple of 8), the layout pool_skip_bwd reads in place; None otherwise."""
+    if t.dtype != torch.bfloat16 or t.dim() != 4:
+        return None
+    N, H, W, C = t.shape
+    ld = t.stride(2)
+    if (t.stride(3) != 1 or ld < C or ld % 8 or t.stride(1) != W * ld or t.stride(0) != H * W * ld
+            or t.storage_offset() % 8):
+        return None
+    return ld
+
+
 def pool_skip_bwd(dp, dskip, s):
     """Gradient of s [N, H, W, C] bf16, a ReLU output feeding both a 2 x 2
     average pool (output gradient dp [N, H/2, W/2, C]) and a skip connection
@@ -236,9 +251,8 @@ def pool_skip_bwd(dp, dskip, s):
         raise ValueError("s must be [N, H, W, C] with H, W even and C a multiple of 8")
     if tuple(dskip.shape) != (N, H, W, C) or tuple(dp.shape) != (N, H // 2, W // 2, C):
         raise ValueError("dskip must be shaped like s and dp like its 2 x 2 pool")
-    ld = dskip.stride(2)
-    if (dskip.dtype != torch.bfloat16 or dskip.stride(3) != 1 or ld < C or ld % 8
-            or dskip.stride(1) != W * ld or dskip.stride(0) != H * W * ld or dskip.storage_offset() % 8):
+    ld = channel_slice_ld(dskip)
+    if ld is None:
         raise ValueError("dskip must be bf16 with contiguous channels, pixels a multiple of 8 apart")
     _same_device(s, dskip, dp)
     out = torch.empty_like(s)

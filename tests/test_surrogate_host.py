@@ -267,3 +267,20 @@ def test_volume_term_of_the_loss():
             assert math.isclose(float(sg.loss_fn(Const(out), x, p0, p1, v, v, wf)) - base, 0.5 * e * e, rel_tol=1e-5)
     finally:
         sg.LAMBDA_V = old
+
+
+def test_channel_slice_layout_of_the_pooled_skip_gradient():
+    """The layouts pool_skip_bwd reads in place (contiguous, or a channel
+    slice at a multiple of 8) and those it does not (any other stride or
+    offset, or not bf16): the backward copies the latter first."""
+    import torch
+    from fingering_dynamics_amd import surrogate_ops as so
+    wide = torch.zeros(2, 4, 6, 40, dtype=torch.bfloat16)
+    assert so.channel_slice_ld(wide) == 40
+    assert so.channel_slice_ld(wide[..., 8:24]) == 40
+    assert so.channel_slice_ld(wide[..., 4:20]) is None          # offset not a multiple of 8
+    assert so.channel_slice_ld(wide[:, :, ::2, :16]) == 80       # every other pixel: still uniform
+    assert so.channel_slice_ld(wide[:, ::2, :, :16]) is None     # rows skipped: not one pixel stride
+    assert so.channel_slice_ld(wide.permute(0, 2, 1, 3)) is None
+    assert so.channel_slice_ld(wide.float()) is None
+    assert so.channel_slice_ld(torch.zeros(2, 4, 6, 12, dtype=torch.bfloat16)) is None   # pixels 12 apart
