@@ -79,16 +79,20 @@ class _Conv(torch.autograd.Function):
         x, wt, y = ctx.saved_tensors
         dy = dy.to(torch.bfloat16)
         dy = dy.contiguous()
-        if ctx.relu and not ctx.dy_masked:
+        mask = ctx.relu and not ctx.dy_masked
+        if not ctx.needs_input_grad[0]:
+            # the weight gradient alone: its staging masks dy on load, one
+            # read of y instead of a masking pass
+            dw, db = so.conv3x3_bwd_weight(x, dy, relu_y=y if mask else None)
+            return None, dw, db, None, None, None, None, None
+        if mask:
             # one masking pass: measured faster here than the kernels'
             # mask-on-load (relu_y), which adds a load per operand fragment,
             # and than a mask in the input-gradient epilogue of the
             # convolution consuming y, whose scattered 2-byte mask loads
             # slowed those kernels by more than the pass costs
             dy = so.relu_mask(dy, y)
-        dx = None
-        if ctx.needs_input_grad[0]:
-            dx = so.conv3x3_bwd_data(dy, wt)
+        dx = so.conv3x3_bwd_data(dy, wt)
         dw, db = so.conv3x3_bwd_weight(x, dy)
         return dx, dw, db, None, None, None, None, None
 
