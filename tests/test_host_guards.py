@@ -103,3 +103,34 @@ def test_convolution_wrappers_check_shapes_before_any_launch():
         so.conv3x3_bwd_weight(x, dy)
     with pytest.raises(ValueError, match="w must be"):
         so.flip_weights(_bf(32, 3, 2, 16))
+
+
+def test_plumbing_wrappers_check_arguments_before_any_launch():
+    """cast_flip, up_cat_bwd's coarse mask and pool_skip_bwd reject bad
+    arguments (and host tensors) with ValueError before any native call."""
+    from fingering_dynamics_amd import surrogate_ops as so
+    w = torch.zeros(32, 3, 3, 16)
+    with pytest.raises(ValueError, match="1 to 16"):
+        so.cast_flip([])
+    with pytest.raises(ValueError, match="1 to 16"):
+        so.cast_flip([w] * 17)
+    with pytest.raises(ValueError, match="float32"):
+        so.cast_flip([w.double()])
+    with pytest.raises(ValueError, match="float32"):
+        so.cast_flip([torch.zeros(32, 3, 2, 16)])
+    with pytest.raises(ValueError, match="same GPU device"):
+        so.cast_flip([w])
+    dout = _bf(2, 8, 8, 48)
+    with pytest.raises(ValueError, match="lo_relu"):
+        so.up_cat_bwd(dout, 32, lo_relu=_bf(2, 4, 4, 16))
+    with pytest.raises(ValueError, match="same GPU device"):
+        so.up_cat_bwd(dout, 32, lo_relu=_bf(2, 4, 4, 32))
+    s = _bf(2, 8, 8, 16)
+    with pytest.raises(ValueError, match="dskip must be shaped"):
+        so.pool_skip_bwd(_bf(2, 4, 4, 16), _bf(2, 8, 8, 8), s)
+    with pytest.raises(ValueError, match="dskip must be shaped"):
+        so.pool_skip_bwd(_bf(2, 8, 8, 16), s, s)
+    with pytest.raises(ValueError, match="contiguous channels"):
+        so.pool_skip_bwd(_bf(2, 4, 4, 16), _bf(2, 8, 8, 20)[..., 2:18], s)
+    with pytest.raises(ValueError, match="same GPU device"):
+        so.pool_skip_bwd(_bf(2, 4, 4, 16), _bf(2, 8, 8, 32)[..., 8:24], s)
